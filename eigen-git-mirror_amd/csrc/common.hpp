@@ -1,0 +1,39 @@
+// common.hpp -- device helpers shared by the LLT kernels (gfx950 / CDNA4 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+// FP64 MFMA 16x16x4.  Lane layout of v_mfma_f64_16x16x4_f64, measured on gfx950:
+//   operand a: lane l supplies a[i = l%16][k = l/16]
+//   operand b: lane l supplies b[k = l/16][j = l%16]
+//   result   : lane l, element r holds d[i = 4r + l/16][j = l%16]
+// The kernels compute C[i][j] += sum_k P[i][k] Q[j][k] by passing Q as `a` and P as `b`:
+// both operands are then loaded as x[row0 + l%16][k0 + l/16], and lane l, element r of
+// the result is C[i0 + l%16][j0 + 4r + l/16] -- 16 consecutive rows of one column per
+// group of 16 lanes, which coalesces on column-major stores.
+__device__ __forceinline__ dvec4 mfma_f64(double q, double p, dvec4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(q, p, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Early-exit test used by every kernel of a factorization: a previous kernel on the
+// stream recorded a failing pivot.  Uniform across the workgroup.
+__device__ __forceinline__ bool factor_failed(const int *info)
+{
+    return info != nullptr && __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// Lower-triangular tile enumeration: t -> (ti, tj), ti >= tj, row-major over the lower
+// triangle (t = ti*(ti+1)/2 + tj).
+__device__ __forceinline__ void lower_tile(int t, int &ti, int &tj)
+{
+    int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    while (i * (i + 1) / 2 > t) --i;
+    ti = i;
+    tj = t - i * (i + 1) / 2;
+}

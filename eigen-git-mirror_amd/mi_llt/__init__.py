@@ -1,0 +1,184 @@
+"""Python binding of the MI355X dense Cholesky (LLT) library.
+
+A thin ctypes layer over the C ABI in ``include/mi_llt.h`` (library
+``eigen-git-mirror_amd/lib/libmi_llt.so``).  PyTorch provides device memory and the
+stream; every computation runs in the HIP kernels of the library.  There is no CPU or
+PyTorch fallback: if the library is missing, :func:`lib` raises.
+
+Matrices are column-major, as Eigen's ``MatrixXd``: a torch tensor ``M`` of shape
+``(n, m)`` with ``M.stride(0) == 1`` holds element (i, j) at ``i + j * ld``,
+``ld = M.stride(1)``.  :func:`colmajor` makes such a tensor.
+
+The object-level mirror of ``Eigen::LLT<MatrixXd>`` (Eigen/src/Cholesky/LLT.h:66-233) is
+:class:`LLT` in ``mi_llt.llt``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  -- must load torch's HIP runtime before the library binds to one
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmi_llt.so")
+
+_lib = None
+
+_I64 = ctypes.c_int64
+_VP = ctypes.c_void_p
+_INT = ctypes.c_int
+
+
+class MiLLTError(RuntimeError):
+    """A status code other than 0 from the C ABI."""
+
+
+def lib() -> ctypes.CDLL:
+    """Load ``libmi_llt.so`` (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MiLLTError(f"{LIB_PATH} not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.mi_llt_create.argtypes = [ctypes.POINTER(_VP)]
+        L.mi_llt_destroy.argtypes = [_VP]
+        L.mi_llt_set_stream.argtypes = [_VP, _VP]
+        L.mi_llt_last_error.argtypes = [_VP]
+        L.mi_llt_last_error.restype = ctypes.c_char_p
+        L.mi_llt_workspace_bytes.argtypes = [_VP]
+        L.mi_llt_workspace_bytes.restype = _I64
+        L.mi_dpotrf.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, ctypes.POINTER(_INT)]
+        L.mi_dpotrf_async.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, _VP]
+        L.mi_dpotrs.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64]
+        L.mi_dposv_async.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64, _VP]
+        L.mi_dsy_l1norm.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, _VP]
+        L.LAPACKE_dpotrf.argtypes = [_INT, ctypes.c_char, _INT, _VP, _INT]
+        L.LAPACKE_dpotrs.argtypes = [_INT, ctypes.c_char, _INT, _INT, _VP, _INT, _VP, _INT]
+        L.dpotrf_.argtypes = [ctypes.c_char_p, ctypes.POINTER(_INT), _VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]
+        L.dpotrs_.argtypes = [ctypes.c_char_p, ctypes.POINTER(_INT), ctypes.POINTER(_INT), _VP,
+                              ctypes.POINTER(_INT), _VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]
+        for name in ("mi_llt_create", "mi_llt_destroy", "mi_llt_set_stream", "mi_dpotrf", "mi_dpotrf_async",
+                     "mi_dpotrs", "mi_dposv_async", "mi_dsy_l1norm", "LAPACKE_dpotrf", "LAPACKE_dpotrs"):
+            getattr(L, name).restype = _INT
+        L.dpotrf_.restype = None
+        L.dpotrs_.restype = None
+        _lib = L
+    return _lib
+
+
+def colmajor(t: torch.Tensor) -> torch.Tensor:
+    """Column-major copy (stride(0) == 1) of a 2-D tensor on its device."""
+    if t.dim() == 1:
+        t = t.reshape(-1, 1)
+    out = torch.empty_strided(tuple(t.shape), (1, max(t.shape[0], 1)), dtype=t.dtype, device=t.device)
+    out.copy_(t)
+    return out
+
+
+def _ld(t: torch.Tensor) -> int:
+    if t.dim() != 2 or (t.shape[0] > 1 and t.stride(0) != 1):
+        raise ValueError("expected a column-major 2-D tensor (stride(0) == 1); use mi_llt.colmajor()")
+    return max(int(t.stride(1)), int(t.shape[0]), 1)
+
+
+class Handle:
+    """Owns a ``mi_llt_handle`` bound to torch's current stream at call time."""
+
+    def __init__(self):
+        h = _VP()
+        rc = lib().mi_llt_create(ctypes.byref(h))
+        if rc != 0:
+            raise MiLLTError(f"mi_llt_create failed ({rc})")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.mi_llt_destroy(self._h)
+            self._h = None
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def bind_stream(self, stream: torch.cuda.Stream | None = None):
+        s = stream if stream is not None else torch.cuda.current_stream()
+        lib().mi_llt_set_stream(self._h, _VP(s.cuda_stream))
+        return self
+
+    def check(self, rc: int, what: str):
+        if rc != 0:
+            msg = lib().mi_llt_last_error(self._h).decode(errors="replace")
+            raise MiLLTError(f"{what} returned {rc}: {msg}")
+
+    def workspace_bytes(self) -> int:
+        return int(lib().mi_llt_workspace_bytes(self._h))
+
+
+_default: Handle | None = None
+
+
+def default_handle() -> Handle:
+    global _default
+    if _default is None:
+        _default = Handle()
+    return _default.bind_stream()
+
+
+def _uplo(u: str) -> bytes:
+    if u not in ("L", "U", "l", "u"):
+        raise ValueError("uplo must be 'L' or 'U'")
+    return u.encode()
+
+
+def potrf_(a: torch.Tensor, uplo: str = "L", handle: Handle | None = None) -> int:
+    """In-place Cholesky of the ``uplo`` triangle of column-major ``a`` (float64, on the GPU).
+
+    Returns LAPACK ``info`` (0, or k+1 for a failing pivot at column k).  Synchronizes.
+    """
+    h = handle.bind_stream() if handle else default_handle()
+    n = int(a.shape[0])
+    info = ctypes.c_int(0)
+    h.check(lib().mi_dpotrf(h.ptr, _uplo(uplo), n, _VP(a.data_ptr()), _ld(a), ctypes.byref(info)), "mi_dpotrf")
+    return int(info.value)
+
+
+def potrf_async_(a: torch.Tensor, d_info: torch.Tensor, uplo: str = "L", handle: Handle | None = None):
+    """Asynchronous :func:`potrf_`; ``d_info`` is a 1-element int32 device tensor."""
+    h = handle.bind_stream() if handle else default_handle()
+    n = int(a.shape[0])
+    h.check(lib().mi_dpotrf_async(h.ptr, _uplo(uplo), n, _VP(a.data_ptr()), _ld(a), _VP(d_info.data_ptr())),
+            "mi_dpotrf_async")
+
+
+def potrs_(l: torch.Tensor, b: torch.Tensor, uplo: str = "L", handle: Handle | None = None):
+    """In-place solve ``b <- A^{-1} b`` with the factor in ``l`` (as left by :func:`potrf_`)."""
+    h = handle.bind_stream() if handle else default_handle()
+    n = int(l.shape[0])
+    nrhs = int(b.shape[1])
+    h.check(lib().mi_dpotrs(h.ptr, _uplo(uplo), n, nrhs, _VP(l.data_ptr()), _ld(l), _VP(b.data_ptr()), _ld(b)),
+            "mi_dpotrs")
+
+
+def posv_async_(a: torch.Tensor, b: torch.Tensor, d_info: torch.Tensor, uplo: str = "L",
+                handle: Handle | None = None):
+    """Factor ``a`` in place and solve ``b`` in place, asynchronously (``d_info`` int32 device)."""
+    h = handle.bind_stream() if handle else default_handle()
+    n = int(a.shape[0])
+    nrhs = int(b.shape[1])
+    h.check(lib().mi_dposv_async(h.ptr, _uplo(uplo), n, nrhs, _VP(a.data_ptr()), _ld(a), _VP(b.data_ptr()),
+                                 _ld(b), _VP(d_info.data_ptr())), "mi_dposv_async")
+
+
+def sy_l1norm(a: torch.Tensor, uplo: str = "L", handle: Handle | None = None) -> torch.Tensor:
+    """``m_l1_norm`` of LLT::compute (LLT.h:442-453) as a 0-d float64 device tensor."""
+    h = handle.bind_stream() if handle else default_handle()
+    out = torch.empty((), dtype=torch.float64, device=a.device)
+    h.check(lib().mi_dsy_l1norm(h.ptr, _uplo(uplo), int(a.shape[0]), _VP(a.data_ptr()), _ld(a),
+                                _VP(out.data_ptr())), "mi_dsy_l1norm")
+    return out
+
+
+from .llt import LLT, ComputationInfo  # noqa: E402
+
+__all__ = ["lib", "colmajor", "Handle", "default_handle", "potrf_", "potrf_async_", "potrs_", "posv_async_",
+           "sy_l1norm", "LLT", "ComputationInfo", "MiLLTError", "LIB_PATH"]

@@ -1,0 +1,91 @@
+/*
+ * mi_llt.h -- C ABI of the MI355X dense Cholesky (LLT) factor + solve library.
+ *
+ * Plain pointers and sizes only.  Matrices are column-major with a leading dimension, as
+ * Eigen's default MatrixXd.  "d" pointers are device (HBM) pointers, and work is queued on
+ * the handle's HIP stream.  Host-pointer drop-ins are in mi_lapacke.h.
+ *
+ * Reference interfaces these entry points replace (Eigen 3.3.90, paths relative to the
+ * reference root):
+ *   internal::llt_inplace<double,Lower|Upper>::blocked   Eigen/src/Cholesky/LLT.h:338-368, 377-399
+ *   LLT::_solve_impl / solveInPlace                      Eigen/src/Cholesky/LLT.h:483-522
+ *   LAPACKE_dpotrf (the EIGEN_USE_LAPACKE plug)          Eigen/src/Cholesky/LLT_LAPACKE.h:42-93
+ *   dpotrf_ / dpotrs_ (Eigen's LAPACK exports)           lapack/cholesky.cpp:14-72
+ *
+ * Status codes: 0 on success; -i when argument i (1-based, counting the handle) is
+ * illegal, as LAPACK's xerbla convention; MI_LLT_E_* below for runtime failures.
+ * Numerical failure is not a status: it is reported through `info` with the LAPACK
+ * convention (0 = success, k+1 = the leading minor of order k+1 is not positive definite,
+ * i.e. Eigen's failing column k, LLT.h:327-330 and lapack/cholesky.cpp:32-33).
+ *
+ * Threading: a handle is not internally synchronised; use one handle per host thread
+ * (handles are independent and may share a device).
+ */
+#ifndef MI_LLT_H
+#define MI_LLT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mi_llt_context *mi_llt_handle;
+
+#define MI_LLT_SUCCESS 0
+#define MI_LLT_E_HIP (-1001)      /* a HIP runtime call failed (see mi_llt_last_error) */
+#define MI_LLT_E_ALLOC (-1002)    /* device workspace allocation failed */
+#define MI_LLT_E_HANDLE (-1003)   /* null or destroyed handle */
+#define MI_LLT_E_UNSUPPORTED (-1004)
+
+/* Trailing-update precision for mi_dposv_mixed. */
+typedef enum {
+    MI_PREC_FP64 = 0,  /* FP64 MFMA trailing update (plain path, no refinement needed) */
+    MI_PREC_FP32 = 1,  /* FP32 factor storage, FP32 MFMA trailing update, FP32 accumulation */
+    MI_PREC_BF16 = 2,  /* FP32 storage, BF16 MFMA trailing update, FP32 accumulation */
+    MI_PREC_BF16X3 = 3 /* FP32 storage, split-BF16 (3 products) trailing update, FP32 acc. */
+} mi_prec;
+
+/* ---- handle ------------------------------------------------------------------------ */
+int mi_llt_create(mi_llt_handle *h);
+int mi_llt_destroy(mi_llt_handle h);
+/* Use `stream` (a hipStream_t) for all later work; NULL selects the null stream. */
+int mi_llt_set_stream(mi_llt_handle h, void *stream);
+/* Text of the last runtime failure on this handle ("" if none). */
+const char *mi_llt_last_error(mi_llt_handle h);
+/* Device bytes currently held by the handle's workspace. */
+int64_t mi_llt_workspace_bytes(mi_llt_handle h);
+
+/* ---- factor ---------------------------------------------------------------------------
+ * In-place Cholesky of the `uplo` triangle ('L' or 'U') of the n x n column-major matrix
+ * at dA.  The other triangle is never read or written (LLT.h:61-62, 225-229).
+ * Replaces internal::llt_inplace<double,Lower|Upper>::blocked (LLT.h:338-368, 377-399),
+ * which LAPACKE_dpotrf replaces under EIGEN_USE_LAPACKE (LLT_LAPACKE.h:59-61).
+ * `info` is a host int; the call waits for the stream. */
+int mi_dpotrf(mi_llt_handle h, char uplo, int64_t n, double *dA, int64_t lda, int *info);
+/* Same, asynchronous: `d_info` is a device int, written with the LAPACK info. */
+int mi_dpotrf_async(mi_llt_handle h, char uplo, int64_t n, double *dA, int64_t lda, int *d_info);
+
+/* ---- solve ----------------------------------------------------------------------------
+ * B <- A^{-1} B with A = L L^T (uplo 'L') or U^T U (uplo 'U') as left in dA by mi_dpotrf.
+ * Replaces LLT::solveInPlace (LLT.h:510-522): forward solve with L then backward with L^T
+ * (triangular_solve_matrix, TriangularSolverMatrix.h:36-184), and dpotrs_
+ * (lapack/cholesky.cpp:41-72).  Asynchronous on the handle's stream. */
+int mi_dpotrs(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double *dA, int64_t lda,
+              double *dB, int64_t ldb);
+
+/* Factor + solve in one call (LLT(A).solve(B), LLT.h:104-110 + 483-498); reuses the
+ * diagonal-block inverses built during the factor.  Asynchronous; `d_info` device int.
+ * On a failed factor B is left unchanged. */
+int mi_dposv_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, double *dA, int64_t lda,
+                   double *dB, int64_t ldb, int *d_info);
+
+/* m_l1_norm of LLT::compute (LLT.h:442-453): max column sum of |.| of the symmetric matrix
+ * whose `uplo` triangle is stored at dA.  Writes one double to the device pointer d_norm. */
+int mi_dsy_l1norm(mi_llt_handle h, char uplo, int64_t n, const double *dA, int64_t lda,
+                  double *d_norm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MI_LLT_H */

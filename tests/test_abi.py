@@ -1,0 +1,102 @@
+"""The C-ABI library loads, exports every symbol the headers declare, and validates
+arguments on the host (no GPU compute is issued by these tests)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import mi_llt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "while", "for", "return", "sizeof", "defined"):
+                names.append(name)
+    return sorted(set(names))
+
+
+def test_headers_declare_the_boundary():
+    names = declared_functions()
+    for must in ("mi_llt_create", "mi_dpotrf", "mi_dpotrs", "mi_dposv_async", "LAPACKE_dpotrf",
+                 "LAPACKE_dpotrs", "dpotrf_", "dpotrs_"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = mi_llt.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_null_handle_is_rejected():
+    lib = mi_llt.lib()
+    info = ctypes.c_int(0)
+    assert lib.mi_dpotrf(None, b"L", 4, None, 4, ctypes.byref(info)) == -1003
+    assert lib.mi_dpotrs(None, b"L", 4, 1, None, 4, None, 4) == -1003
+    assert lib.mi_llt_set_stream(None, None) == -1003
+    assert lib.mi_llt_last_error(None) == b"invalid handle"
+
+
+def test_lapacke_argument_checks():
+    lib = mi_llt.lib()
+    # LAPACKE: -i for illegal argument i; nothing reaches the device
+    assert lib.LAPACKE_dpotrf(100, b"L", 3, None, 3) == -1
+    assert lib.LAPACKE_dpotrf(102, b"X", 3, None, 3) == -2
+    assert lib.LAPACKE_dpotrf(102, b"L", -1, None, 3) == -3
+    assert lib.LAPACKE_dpotrf(102, b"L", 3, None, 2) == -4
+    buf = (ctypes.c_double * 9)()
+    assert lib.LAPACKE_dpotrf(102, b"L", 3, ctypes.cast(buf, ctypes.c_void_p), 2) == -5
+    assert lib.LAPACKE_dpotrf(102, b"L", 0, ctypes.cast(buf, ctypes.c_void_p), 1) == 0
+    assert lib.LAPACKE_dpotrs(102, b"L", 3, -1, None, 3, None, 3) == -4
+    assert lib.LAPACKE_dpotrs(102, b"L", 3, 1, None, 2, None, 3) == -6
+
+
+def test_fortran_argument_checks():
+    # lapack/cholesky.cpp:18-21: uplo -> -1, n -> -2, lda -> -4
+    lib = mi_llt.lib()
+    info = ctypes.c_int(99)
+    n = ctypes.c_int(3)
+    lda = ctypes.c_int(3)
+    lib.dpotrf_(b"Q", ctypes.byref(n), None, ctypes.byref(lda), ctypes.byref(info))
+    assert info.value == -1
+    nneg = ctypes.c_int(-2)
+    lib.dpotrf_(b"L", ctypes.byref(nneg), None, ctypes.byref(lda), ctypes.byref(info))
+    assert info.value == -2
+    lda2 = ctypes.c_int(2)
+    lib.dpotrf_(b"U", ctypes.byref(n), None, ctypes.byref(lda2), ctypes.byref(info))
+    assert info.value == -4
+    nrhs = ctypes.c_int(-1)
+    lib.dpotrs_(b"L", ctypes.byref(n), ctypes.byref(nrhs), None, ctypes.byref(lda), None, ctypes.byref(lda),
+                ctypes.byref(info))
+    assert info.value == -3
+    nrhs = ctypes.c_int(1)
+    lib.dpotrs_(b"L", ctypes.byref(n), ctypes.byref(nrhs), None, ctypes.byref(lda), None, ctypes.byref(lda2),
+                ctypes.byref(info))
+    assert info.value == -7
+
+
+def test_colmajor_helper_layout():
+    import torch
+
+    t = torch.arange(6.0, dtype=torch.float64).reshape(2, 3)
+    c = mi_llt.colmajor(t)
+    assert c.stride() == (1, 2) and torch.equal(c, t)
+
+
+def test_product_path_has_no_oracle_import():
+    pkg = os.path.join(ROOT, "eigen-git-mirror_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".hpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "llt_oracle" not in text, f
