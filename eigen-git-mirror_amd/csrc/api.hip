@@ -11,19 +11,41 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <new>
+#include <vector>
+
 #include <hip/hip_runtime.h>
 
 #include "../../include/mi_llt.h"
 #include "kernels.hpp"
 
 struct mi_llt_context {
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // caller's stream: all work is ordered on it
+    hipStream_t panel = nullptr;        // internal stream for the lookahead panel work
+    std::vector<hipEvent_t> events;     // per-step dependency events (grow-only pool)
     int device = 0;
     void *ws = nullptr;       // grow-only device workspace
     size_t ws_bytes = 0;
     int *d_info = nullptr;    // device int for the synchronous entry points
+    // persistent synchronisation words of the solve: [status | 2 tickets | flags...]
+    uint32_t *sync = nullptr;
+    size_t sync_words = 0;
+    uint32_t epoch = 0;          // solve flags
+    uint32_t panel_epoch = 0;    // panel-step flag
+    uint32_t panel_tickets = 0;  // tickets drawn so far from the panel ticket counter
     char err[256] = {0};
     uint32_t magic = 0x4c4c5431u;
+    // optional per-kernel timing (mi_llt_trace_*): event pairs around each launch
+    bool trace = false;
+    struct Rec {
+        int kind;
+        double flops;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Rec> recs;
+    double t_ms[MI_TRACE_KINDS] = {0};
+    double t_flops[MI_TRACE_KINDS] = {0};
+    int64_t t_calls[MI_TRACE_KINDS] = {0};
 };
 
 namespace {
@@ -43,6 +65,33 @@ int uplo_code(char uplo)
     if (uplo == 'L' || uplo == 'l') return 0;
     if (uplo == 'U' || uplo == 'u') return 1;
     return -1;
+}
+
+// Run `launch` (which enqueues on stream s), bracketed by events when tracing is on.
+template <class F>
+void traced(mi_llt_handle h, hipStream_t s, int kind, double flops, F &&launch)
+{
+    if (!h->trace) {
+        launch();
+        return;
+    }
+    mi_llt_context::Rec r{kind, flops, nullptr, nullptr};
+    hipEventCreate(&r.e0);
+    hipEventCreate(&r.e1);
+    hipEventRecord(r.e0, s);
+    launch();
+    hipEventRecord(r.e1, s);
+    h->recs.push_back(r);
+}
+
+int ensure_events(mi_llt_handle h, size_t count)
+{
+    while (h->events.size() < count) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return MI_LLT_E_HIP;
+        h->events.push_back(e);
+    }
+    return 0;
 }
 
 size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -72,37 +121,116 @@ int check_launch(mi_llt_handle h, const char *what)
     return e == hipSuccess ? 0 : fail_hip(h, e, what);
 }
 
-size_t inv_bytes(int64_t n)
-{
-    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
-    return round_up((size_t)nblk * mi::kNB * mi::kNB * sizeof(double));
-}
-
-// Workspace layout: [ Winv | lower copy for uplo='U' (n*n) ]
+// Workspace layout: [ Wleaf | Winv | lower copy for uplo='U' (n*n) ]
 struct Layout {
+    double *wleaf;
     double *winv;
     double *tri;
 };
 
 int prepare(mi_llt_handle h, int64_t n, bool upper, Layout &lay)
 {
-    size_t need = inv_bytes(n);
-    size_t tri_off = need;
+    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
+    const size_t leaf = round_up((size_t)nblk * mi::kLeafDoubles * sizeof(double));
+    const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
+    size_t need = leaf + inv;
     if (upper) need += round_up((size_t)n * n * sizeof(double));
     int rc = reserve(h, need);
     if (rc) return rc;
-    lay.winv = (double *)h->ws;
-    lay.tri = upper ? (double *)((char *)h->ws + tri_off) : nullptr;
+    lay.wleaf = (double *)h->ws;
+    lay.winv = (double *)((char *)h->ws + leaf);
+    lay.tri = upper ? (double *)((char *)h->ws + leaf + inv) : nullptr;
     return 0;
 }
 
-void factor_lower(double *A, int64_t lda, int64_t n, double *winv, int *d_info, hipStream_t s)
+constexpr size_t kSyncHeader = 8;
+
+int reserve_sync(mi_llt_handle h, size_t flag_words)
 {
-    for (int64_t k0 = 0; k0 < n; k0 += mi::kNB) {
-        mi::launch_potrf_diag(A, lda, n, k0, winv, d_info, s);
-        mi::launch_trsm_panel(A, lda, n, k0, winv, d_info, s);
-        mi::launch_syrk_update(A, lda, n, k0, d_info, s);
+    const size_t need = kSyncHeader + flag_words;
+    if (need <= h->sync_words) return 0;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
+    if (h->sync) hipFree(h->sync);
+    h->sync = nullptr;
+    h->sync_words = 0;
+    const size_t words = need < 4096 ? 4096 : need;
+    if (hipMalloc(&h->sync, words * sizeof(uint32_t)) != hipSuccess) return MI_LLT_E_ALLOC;
+    e = hipMemset(h->sync, 0, words * sizeof(uint32_t));
+    if (e != hipSuccess) return fail_hip(h, e, "hipMemset");
+    h->sync_words = words;
+    h->panel_epoch = 0;
+    h->panel_tickets = 0;
+    return 0;
+}
+
+// Algorithmic flop counts (LAPACK convention): POTRF of b: b^3/3 + b^2/2 + b/6;
+// TRSM m x b panel: m b^2; SYRK lower m x m, depth b: m (m + 1) b; panel update m x b x b.
+//
+// Right-looking blocked Cholesky (llt_inplace::blocked, LLT.h:338-368) with a one-panel
+// lookahead, one launch per step on the caller's stream:
+//   launch 0   : panel 0 (diagonal factor + panel solve of block column 0)
+//   launch k+1 : panel k+1 (its update by panel k, diagonal factor, panel solve)
+//                + trailing update by panel k of the columns from (k+2)*NB on.
+// The two halves of a launch touch disjoint data; everything they read was finished by an
+// earlier launch, so stream order is the only synchronisation the host needs.
+double panel_flops(int64_t n, int64_t k0, int64_t NB)
+{
+    const double b = (double)((n - k0) < NB ? (n - k0) : NB);
+    const double m = (double)(n - k0 - NB > 0 ? n - k0 - NB : 0);
+    const double mc = (double)(n - k0);
+    return (k0 > 0 ? 2.0 * mc * b * NB - b * (b - 1) * NB : 0.0) + b * b * b / 3 + b * b / 2 + b / 6 + m * b * b;
+}
+
+int factor_lower(mi_llt_handle h, double *A, int64_t lda, int64_t n, double *wleaf, int *d_info)
+{
+    constexpr int64_t NB = mi::kNB;
+    const int64_t nblk = (n + NB - 1) / NB;
+    int rc = reserve_sync(h, 0);
+    if (rc) return rc;
+    if (h->panel_epoch > 0x7fff0000u) {  // the flag only grows: restart it from zero
+        hipError_t e = hipMemsetAsync(h->sync + 5, 0, sizeof(uint32_t), h->stream);
+        if (e != hipSuccess) return fail_hip(h, e, "hipMemsetAsync");
+        h->panel_epoch = 0;
     }
+    hipStream_t s = h->stream;
+    for (int64_t k = -1; k < nblk; ++k) {
+        const int64_t kp = (k + 1 < nblk) ? (k + 1) * NB : -1;  // panel of this launch
+        const int64_t ks = (k >= 0 && n - k * NB - 2 * NB > 0) ? k * NB : -1;  // trailing update
+        if (kp < 0 && ks < 0) break;
+        const double m2 = ks >= 0 ? (double)(n - ks - 2 * NB) : 0.0;
+        const double fl = (kp >= 0 ? panel_flops(n, kp, NB) : 0.0) + m2 * (m2 + 1) * NB;
+        const uint32_t epoch = (kp >= 0) ? ++h->panel_epoch : h->panel_epoch;
+        uint32_t grid = 0;
+        traced(h, s, MI_TRACE_POTRF, fl, [&] {
+            mi::launch_factor_step(A, lda, n, kp, ks, wleaf, d_info, h->sync + 4, h->panel_tickets, h->sync + 5, epoch,
+                                   (int *)h->sync, s, &grid);
+        });
+        h->panel_tickets += grid;
+    }
+    return 0;
+}
+
+int solve_lower(mi_llt_handle h, const double *L, int64_t ldl, int64_t n, double *B, int64_t ldb, int64_t nrhs,
+                const double *wleaf, double *winv, const int *d_info)
+{
+    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
+    const int64_t ngrp = (nrhs + 15) / 16;
+    int rc = reserve_sync(h, (size_t)(nblk * ngrp));
+    if (rc) return rc;
+    if (h->epoch > 0x7fffffffu) {  // epochs only grow: start over on fresh (zero) flags
+        hipError_t e = hipMemsetAsync(h->sync + kSyncHeader, 0, (h->sync_words - kSyncHeader) * 4, h->stream);
+        if (e != hipSuccess) return fail_hip(h, e, "hipMemsetAsync");
+        h->epoch = 0;
+    }
+    const uint32_t ef = ++h->epoch, eb = ++h->epoch;
+    traced(h, h->stream, MI_TRACE_TRTRI, (double)n * mi::kNB * mi::kNB / 3.0,
+           [&] { mi::launch_block_inverses(L, ldl, n, wleaf, winv, d_info, h->stream); });
+    traced(h, h->stream, MI_TRACE_SOLVE, 2.0 * (double)n * (double)n * (double)nrhs, [&] {
+        mi::launch_solve_persistent(L, ldl, n, B, ldb, nrhs, winv, h->sync + kSyncHeader, (int *)h->sync + 1, ef,
+                                    eb, d_info, (int *)h->sync, h->stream);
+    });
+    return 0;
 }
 
 }  // namespace
@@ -119,6 +247,12 @@ int mi_llt_create(mi_llt_handle *h)
         delete c;
         return MI_LLT_E_ALLOC;
     }
+    // default priority (no stream priorities anywhere)
+    if (hipStreamCreateWithFlags(&c->panel, hipStreamNonBlocking) != hipSuccess) {
+        hipFree(c->d_info);
+        delete c;
+        return MI_LLT_E_HIP;
+    }
     *h = c;
     return 0;
 }
@@ -127,6 +261,10 @@ int mi_llt_destroy(mi_llt_handle h)
 {
     if (!valid(h)) return MI_LLT_E_HANDLE;
     hipStreamSynchronize(h->stream);
+    hipStreamSynchronize(h->panel);
+    for (hipEvent_t e : h->events) hipEventDestroy(e);
+    if (h->sync) hipFree(h->sync);
+    hipStreamDestroy(h->panel);
     if (h->ws) hipFree(h->ws);
     if (h->d_info) hipFree(h->d_info);
     h->magic = 0;
@@ -145,6 +283,62 @@ const char *mi_llt_last_error(mi_llt_handle h) { return valid(h) ? h->err : "inv
 
 int64_t mi_llt_workspace_bytes(mi_llt_handle h) { return valid(h) ? (int64_t)h->ws_bytes : -1; }
 
+int mi_llt_status(mi_llt_handle h, int *status)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    if (!status) return -2;
+    *status = 0;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
+    if (!h->sync) return 0;
+    int v = 0;
+    e = hipMemcpy(&v, h->sync, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && v != 0) e = hipMemset(h->sync, 0, sizeof(int));
+    if (e != hipSuccess) return fail_hip(h, e, "status readback");
+    *status = v;
+    return 0;
+}
+
+int mi_llt_trace(mi_llt_handle h, int enable)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    h->trace = enable != 0;
+    return 0;
+}
+
+int mi_llt_trace_read(mi_llt_handle h, int kind, double *ms, double *flops, int64_t *launches)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    if (kind < 0 || kind >= MI_TRACE_KINDS) return -2;
+    if (!h->recs.empty()) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return fail_hip(h, e, "trace sync");
+        for (auto &r : h->recs) {
+            float t = 0.f;
+            hipEventElapsedTime(&t, r.e0, r.e1);
+            h->t_ms[r.kind] += t;
+            h->t_flops[r.kind] += r.flops;
+            h->t_calls[r.kind] += 1;
+            hipEventDestroy(r.e0);
+            hipEventDestroy(r.e1);
+        }
+        h->recs.clear();
+    }
+    if (ms) *ms = h->t_ms[kind];
+    if (flops) *flops = h->t_flops[kind];
+    if (launches) *launches = h->t_calls[kind];
+    return 0;
+}
+
+int mi_llt_trace_reset(mi_llt_handle h)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    double ms;
+    mi_llt_trace_read(h, 0, &ms, nullptr, nullptr);
+    for (int k = 0; k < MI_TRACE_KINDS; ++k) h->t_ms[k] = h->t_flops[k] = 0.0, h->t_calls[k] = 0;
+    return 0;
+}
+
 int mi_dpotrf_async(mi_llt_handle h, char uplo, int64_t n, double *dA, int64_t lda, int *d_info)
 {
     if (!valid(h)) return MI_LLT_E_HANDLE;
@@ -161,10 +355,12 @@ int mi_dpotrf_async(mi_llt_handle h, char uplo, int64_t n, double *dA, int64_t l
     int rc = prepare(h, n, up == 1, lay);
     if (rc) return rc;
     if (up == 0) {
-        factor_lower(dA, lda, n, lay.winv, d_info, s);
+        rc = factor_lower(h, dA, lda, n, lay.wleaf, d_info);
+        if (rc) return rc;
     } else {
         mi::launch_upper_to_lower(dA, lda, n, lay.tri, n, s);
-        factor_lower(lay.tri, n, n, lay.winv, d_info, s);
+        rc = factor_lower(h, lay.tri, n, n, lay.wleaf, d_info);
+        if (rc) return rc;
         mi::launch_lower_to_upper(lay.tri, n, n, dA, lda, s);
     }
     return check_launch(h, "factor");
@@ -205,8 +401,9 @@ int mi_dpotrs(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double 
         L = lay.tri;
         ldl = n;
     }
-    mi::launch_trtri_blocks(L, ldl, n, lay.winv, s);
-    mi::launch_solve(L, ldl, n, dB, ldb, nrhs, lay.winv, nullptr, s);
+    mi::launch_leaf_inverses(L, ldl, n, lay.wleaf, s);
+    rc = solve_lower(h, L, ldl, n, dB, ldb, nrhs, lay.wleaf, lay.winv, nullptr);
+    if (rc) return rc;
     return check_launch(h, "solve");
 }
 
@@ -236,8 +433,12 @@ int mi_dposv_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, double *
         L = lay.tri;
         ldl = n;
     }
-    factor_lower(L, ldl, n, lay.winv, d_info, s);
-    if (nrhs > 0) mi::launch_solve(L, ldl, n, dB, ldb, nrhs, lay.winv, d_info, s);
+    rc = factor_lower(h, L, ldl, n, lay.wleaf, d_info);
+    if (rc) return rc;
+    if (nrhs > 0) {
+        rc = solve_lower(h, L, ldl, n, dB, ldb, nrhs, lay.wleaf, lay.winv, d_info);
+        if (rc) return rc;
+    }
     if (up == 1) mi::launch_lower_to_upper(lay.tri, n, n, dA, lda, s);
     return check_launch(h, "posv");
 }

@@ -31,9 +31,9 @@ __device__ __forceinline__ bool factor_failed(const int *info)
 // triangle (t = ti*(ti+1)/2 + tj).
 __device__ __forceinline__ void lower_tile(int t, int &ti, int &tj)
 {
-    int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    int i = (int)((__fsqrt_rn(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);  // estimate, then exact fix-up
     while ((i + 1) * (i + 2) / 2 <= t) ++i;
-    while (i * (i + 1) / 2 > t) --i;
+    while (i > 0 && i * (i + 1) / 2 > t) --i;
     ti = i;
     tj = t - i * (i + 1) / 2;
 }

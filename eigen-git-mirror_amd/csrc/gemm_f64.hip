@@ -4,373 +4,66 @@
 //       replaces the trailing rank update of llt_inplace::blocked, LLT.h:365
 //       (SelfadjointProduct.h:82-117 -> GeneralMatrixMatrixTriangular.h:57-194, whose cycles
 //       are spent in gebp_kernel, GeneralBlockPanelKernel.h:1548-2217).
-//   trsm_panel_kernel  A21 <- A21 * X^T with X = L11^{-1}
-//       replaces A11^T.triangularView<Upper>().solveInPlace<OnTheRight>(A21), LLT.h:364
-//       (triangular_solve_matrix<..,OnTheRight,..>, TriangularSolverMatrix.h:186-329).
-//   solve_diag_kernel / solve_update_kernel
-//       block forward/backward substitution of LLT::_solve_impl (LLT.h:483-498,
-//       TriangularSolverMatrix.h:36-184): Y_b = X_b * B_b, then B_rest -= L_rest,b * Y_b.
 //
-// All products have the form C[i][j] (+)= sum_k P[i][k] * Q[j][k] and run on
-// v_mfma_f64_16x16x4_f64.  Operand tiles are staged global -> registers -> LDS in chunks
-// of KC = 16 along k; the next chunk is fetched into registers while the current one feeds
-// the MFMAs.
+// Products have the form C[i][j] -= sum_k P[i][k] * Q[j][k] on v_mfma_f64_16x16x4_f64, with
+// the operands read by each wave straight from L2 (mfma_tile.hpp).
 #include "common.hpp"
 #include "kernels.hpp"
+#include "syrk_device.hpp"
 
 namespace mi {
 
-constexpr int KC = 16;   // k-chunk staged through LDS
-constexpr int PAD = 16;  // LDS row padding (doubles): rows of a chunk land in alternate bank halves
-
-// KC x ROWS operand tile.  Source element (r, k) is at g[r * rs + k * ks]; elements with
-// r >= rmax or k >= kmax read as 0.  KCONTIG selects the lane mapping that keeps global
-// reads contiguous when ks == 1 (else rs == 1 is assumed).
-template <int ROWS, bool KCONTIG>
-struct Tile {
-    static constexpr int PER = ROWS * KC / 256;
-    double v[PER];
-
-    __device__ __forceinline__ static void coords(int q, int &r, int &k)
-    {
-        const int e = q * 256 + (int)threadIdx.x;
-        if (KCONTIG) {
-            k = e % KC;
-            r = e / KC;
-        } else {
-            r = e % ROWS;
-            k = e / ROWS;
-        }
-    }
-    __device__ __forceinline__ void fetch(const double *g, int64_t rs, int64_t ks, int rmax, int kmax)
-    {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int r, k;
-            coords(q, r, k);
-            v[q] = (r < rmax && k < kmax) ? g[r * rs + k * ks] : 0.0;
-        }
-    }
-    __device__ __forceinline__ void stash(double *s) const
-    {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int r, k;
-            coords(q, r, k);
-            s[k * (ROWS + PAD) + r] = v[q];
-        }
-    }
-};
-
 // ------------------------------------------------------------------------------ SYRK
-// One 128x128 tile of the lower triangle of A22 per workgroup; 4 waves in a 2x2 grid of
-// 64x64 sub-tiles (4x4 MFMA tiles each).  On diagonal tiles the strictly-upper 64x64
-// quadrant is skipped.
-__global__ __launch_bounds__(256) void syrk_lower_kernel(double *__restrict__ A, int64_t lda, int64_t n,
-                                                         int64_t k0, const int *info)
+// C (m x m, lower triangle) -= P P^T, P = m x K, one 128 x 128 tile per workgroup
+// (syrk_device.hpp).  FIRST_COL restricts the update to tile column 0 and to columns
+// j < ncol (the next panel's columns).
+template <bool FIRST_COL, int K>
+__global__ __launch_bounds__(256) void syrk_tiles_kernel(double *__restrict__ C, int64_t ldc,
+                                                         const double *__restrict__ P, int64_t ldp, int m,
+                                                         int ncol, const int *info)
 {
     if (factor_failed(info)) return;
-    const int64_t r0 = k0 + kNB;
-    const int rs = (int)(n - r0);
     int ti, tj;
-    lower_tile((int)blockIdx.x, ti, tj);
-    const int i0 = ti * 128, j0 = tj * 128;
-    const double *P = A + r0 + k0 * lda;
-
-    __shared__ double Ps[KC * (128 + PAD)];
-    __shared__ double Qs[KC * (128 + PAD)];
-    const int w = threadIdx.x >> 6, l = lane_id();
-    const int wr = w >> 1, wc = w & 1;
-    const bool active = !(ti == tj && wr == 0 && wc == 1);
-
-    dvec4 acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
-
-    Tile<128, false> tp, tq;
-    tp.fetch(P + i0, 1, lda, rs - i0, KC);
-    tq.fetch(P + j0, 1, lda, rs - j0, KC);
-    for (int kc = 0; kc < kNB; kc += KC) {
-        __syncthreads();
-        tp.stash(Ps);
-        tq.stash(Qs);
-        __syncthreads();
-        if (kc + KC < kNB) {
-            tp.fetch(P + i0 + (kc + KC) * lda, 1, lda, rs - i0, KC);
-            tq.fetch(P + j0 + (kc + KC) * lda, 1, lda, rs - j0, KC);
-        }
-        if (active) {
-#pragma unroll
-            for (int s = 0; s < KC / 4; ++s) {
-                const int kr = (s * 4 + (l >> 4)) * (128 + PAD);
-                double pf[4], qf[4];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) pf[a] = Ps[kr + wr * 64 + a * 16 + (l & 15)];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) qf[b] = Qs[kr + wc * 64 + b * 16 + (l & 15)];
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] = mfma_f64(qf[b], pf[a], acc[a][b]);
-            }
-        }
+    if (FIRST_COL) {
+        ti = blockIdx.x;
+        tj = 0;
+    } else {
+        const int T = (m + 127) / 128;
+        lower_tile(xcd_remap((int)blockIdx.x, T * (T + 1) / 2), ti, tj);
     }
-    if (!active) return;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wr * 64 + a * 16 + (l & 15);
-                const int j = j0 + wc * 64 + b * 16 + 4 * r + (l >> 4);
-                if (i < rs && j <= i) A[(r0 + i) + (r0 + j) * lda] -= acc[a][b][r];
-            }
+    syrk_tile<K>(C, ldc, P, ldp, m, ncol, ti, tj);
+}
+
+// K is the depth of the update: 128 (one panel) or 256 (two panels).
+void launch_syrk(double *C, int64_t ldc, const double *P, int64_t ldp, int64_t m, int64_t K, const int *info,
+                 hipStream_t s)
+{
+    if (m <= 0) return;
+    const int T = (int)((m + 127) / 128);
+    const dim3 g(T * (T + 1) / 2);
+    if (K == 256)
+        hipLaunchKernelGGL((syrk_tiles_kernel<false, 256>), g, dim3(256), 0, s, C, ldc, P, ldp, (int)m, (int)m, info);
+    else
+        hipLaunchKernelGGL((syrk_tiles_kernel<false, 128>), g, dim3(256), 0, s, C, ldc, P, ldp, (int)m, (int)m, info);
+}
+
+void launch_syrk_first_col(double *C, int64_t ldc, const double *P, int64_t ldp, int64_t m, int64_t ncol,
+                           int64_t K, const int *info, hipStream_t s)
+{
+    if (m <= 0 || ncol <= 0) return;
+    const int T = (int)((m + 127) / 128);
+    if (K == 256)
+        hipLaunchKernelGGL((syrk_tiles_kernel<true, 256>), dim3(T), dim3(256), 0, s, C, ldc, P, ldp, (int)m,
+                           (int)ncol, info);
+    else
+        hipLaunchKernelGGL((syrk_tiles_kernel<true, 128>), dim3(T), dim3(256), 0, s, C, ldc, P, ldp, (int)m,
+                           (int)ncol, info);
 }
 
 void launch_syrk_update(double *A, int64_t lda, int64_t n, int64_t k0, const int *info, hipStream_t s)
 {
-    const int64_t rs = n - k0 - kNB;
-    if (rs <= 0) return;
-    const int T = (int)((rs + 127) / 128);
-    hipLaunchKernelGGL(syrk_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, A, lda, n, k0, info);
-}
-
-// ------------------------------------------------------------------------------ panel TRSM
-// 64 panel rows x kNB columns per workgroup; wave w owns columns 32w..32w+31.  The
-// workgroup reads all of its rows before writing them, so the update is in place.
-__global__ __launch_bounds__(256) void trsm_panel_kernel(double *__restrict__ A, int64_t lda, int64_t n,
-                                                         int64_t k0, const double *__restrict__ Winv,
-                                                         const int *info)
-{
-    if (factor_failed(info)) return;
     const int64_t r0 = k0 + kNB;
-    const int rs = (int)(n - r0);
-    const int i0 = blockIdx.x * 64;
-    double *P = A + r0 + k0 * lda;
-    const double *X = Winv + (size_t)(k0 / kNB) * kNB * kNB;
-
-    __shared__ double Ps[KC * (64 + PAD)];
-    __shared__ double Qs[KC * (128 + PAD)];
-    const int w = threadIdx.x >> 6, l = lane_id();
-
-    dvec4 acc[4][2];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
-
-    Tile<64, false> tp;
-    Tile<128, false> tq;
-    tp.fetch(P + i0, 1, lda, rs - i0, KC);
-    tq.fetch(X, 1, kNB, kNB, KC);
-    for (int kc = 0; kc < kNB; kc += KC) {
-        __syncthreads();
-        tp.stash(Ps);
-        tq.stash(Qs);
-        __syncthreads();
-        if (kc + KC < kNB) {
-            tp.fetch(P + i0 + (kc + KC) * lda, 1, lda, rs - i0, KC);
-            tq.fetch(X + (kc + KC) * kNB, 1, kNB, kNB, KC);
-        }
-#pragma unroll
-        for (int s = 0; s < KC / 4; ++s) {
-            const int kr = s * 4 + (l >> 4);
-            double pf[4], qf[2];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) pf[a] = Ps[kr * (64 + PAD) + a * 16 + (l & 15)];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) qf[b] = Qs[kr * (128 + PAD) + w * 32 + b * 16 + (l & 15)];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f64(qf[b], pf[a], acc[a][b]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = i0 + a * 16 + (l & 15);
-                const int j = w * 32 + b * 16 + 4 * r + (l >> 4);
-                if (i < rs) P[i + j * lda] = acc[a][b][r];
-            }
-}
-
-void launch_trsm_panel(double *A, int64_t lda, int64_t n, int64_t k0, const double *Winv, const int *info,
-                       hipStream_t s)
-{
-    const int64_t rs = n - k0 - kNB;
-    if (rs <= 0) return;
-    hipLaunchKernelGGL(trsm_panel_kernel, dim3((unsigned)((rs + 63) / 64)), dim3(256), 0, s, A, lda, n, k0,
-                       Winv, info);
-}
-
-// ------------------------------------------------------------------------------ solve
-// Y_b = op(X_b) * B_b for the rows of block b (op = X for the forward sweep with L,
-// X^T for the backward sweep with L^T).  kNB rows x 64 right-hand sides per workgroup;
-// wave w owns rows 32w..32w+31.  In place on B.
-template <bool TRANS>
-__global__ __launch_bounds__(256) void solve_diag_kernel(double *__restrict__ B, int64_t ldb, int64_t n,
-                                                         int64_t nrhs, int64_t k0,
-                                                         const double *__restrict__ Winv, const int *info)
-{
-    if (factor_failed(info)) return;
-    const int bs = (int)((n - k0) < kNB ? (n - k0) : kNB);
-    const int c0 = blockIdx.x * 64;
-    const int nc = (int)(nrhs - c0);
-    const double *X = Winv + (size_t)(k0 / kNB) * kNB * kNB;
-    double *Bb = B + k0 + (int64_t)c0 * ldb;
-
-    __shared__ double Ps[KC * (128 + PAD)];
-    __shared__ double Qs[KC * (64 + PAD)];
-    const int w = threadIdx.x >> 6, l = lane_id();
-
-    dvec4 acc[2][4];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
-
-    // P[i][k] = X[i][k] (forward) or X[k][i] (backward); Q[c][k] = B[k0 + k][c0 + c]
-    Tile<128, TRANS> tp;
-    Tile<64, true> tq;
-    const int64_t prs = TRANS ? kNB : 1, pks = TRANS ? 1 : kNB;
-    tp.fetch(X, prs, pks, kNB, KC);
-    tq.fetch(Bb, ldb, 1, nc, bs);
-    for (int kc = 0; kc < kNB; kc += KC) {
-        __syncthreads();
-        tp.stash(Ps);
-        tq.stash(Qs);
-        __syncthreads();
-        if (kc + KC < kNB) {
-            tp.fetch(X + (kc + KC) * pks, prs, pks, kNB, KC);
-            tq.fetch(Bb + (kc + KC), ldb, 1, nc, bs - (kc + KC));
-        }
-#pragma unroll
-        for (int s = 0; s < KC / 4; ++s) {
-            const int kr = s * 4 + (l >> 4);
-            double pf[2], qf[4];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) pf[a] = Ps[kr * (128 + PAD) + w * 32 + a * 16 + (l & 15)];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) qf[b] = Qs[kr * (64 + PAD) + b * 16 + (l & 15)];
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = mfma_f64(qf[b], pf[a], acc[a][b]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = w * 32 + a * 16 + (l & 15);
-                const int c = b * 16 + 4 * r + (l >> 4);
-                if (i < bs && c < nc) Bb[i + (int64_t)c * ldb] = acc[a][b][r];
-            }
-}
-
-// Forward (TRANS=false): B[rows below block] -= L[rows, block] * Y_b.
-// Backward (TRANS=true): B[rows above block] -= L[block, rows]^T * Y_b.
-// 64 rows x 64 right-hand sides per workgroup, 2x2 waves of 32x32.
-template <bool TRANS>
-__global__ __launch_bounds__(256) void solve_update_kernel(const double *__restrict__ L, int64_t lda,
-                                                           int64_t n, double *__restrict__ B, int64_t ldb,
-                                                           int64_t nrhs, int64_t k0, const int *info)
-{
-    if (factor_failed(info)) return;
-    const int bs = (int)((n - k0) < kNB ? (n - k0) : kNB);
-    const int64_t rbase = TRANS ? 0 : k0 + kNB;
-    const int m = (int)(TRANS ? k0 : n - rbase);
-    const int i0 = blockIdx.x * 64;
-    const int c0 = blockIdx.y * 64;
-    const int nc = (int)(nrhs - c0);
-
-    // P[i][k] = L[rbase + i0 + i][k0 + k] (forward) or L[k0 + k][i0 + i] (backward)
-    const double *Pg = TRANS ? (L + k0 + (i0 + rbase) * lda) : (L + (rbase + i0) + k0 * lda);
-    const int64_t prs = TRANS ? lda : 1, pks = TRANS ? 1 : lda;
-    const double *Qg = B + k0 + (int64_t)c0 * ldb;  // Q[c][k] = Y[k][c]
-
-    __shared__ double Ps[KC * (64 + PAD)];
-    __shared__ double Qs[KC * (64 + PAD)];
-    const int w = threadIdx.x >> 6, l = lane_id();
-    const int wr = w >> 1, wc = w & 1;
-
-    dvec4 acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
-
-    Tile<64, TRANS> tp;
-    Tile<64, true> tq;
-    tp.fetch(Pg, prs, pks, m - i0, bs);
-    tq.fetch(Qg, ldb, 1, nc, bs);
-    for (int kc = 0; kc < bs; kc += KC) {
-        __syncthreads();
-        tp.stash(Ps);
-        tq.stash(Qs);
-        __syncthreads();
-        if (kc + KC < bs) {
-            tp.fetch(Pg + (kc + KC) * pks, prs, pks, m - i0, bs - (kc + KC));
-            tq.fetch(Qg + (kc + KC), ldb, 1, nc, bs - (kc + KC));
-        }
-#pragma unroll
-        for (int s = 0; s < KC / 4; ++s) {
-            const int kr = (s * 4 + (l >> 4)) * (64 + PAD);
-            double pf[2], qf[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) pf[a] = Ps[kr + wr * 32 + a * 16 + (l & 15)];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) qf[b] = Qs[kr + wc * 32 + b * 16 + (l & 15)];
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f64(qf[b], pf[a], acc[a][b]);
-        }
-    }
-    double *Bo = B + rbase + i0 + (int64_t)c0 * ldb;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = wr * 32 + a * 16 + (l & 15);
-                const int c = wc * 32 + b * 16 + 4 * r + (l >> 4);
-                if (i0 + i < m && c < nc) Bo[i + (int64_t)c * ldb] -= acc[a][b][r];
-            }
-}
-
-void launch_solve(const double *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
-                  const double *Winv, const int *info, hipStream_t s)
-{
-    const int64_t nblk = (n + kNB - 1) / kNB;
-    const unsigned gc = (unsigned)((nrhs + 63) / 64);
-    for (int64_t b = 0; b < nblk; ++b) {  // forward: L Y = B
-        const int64_t k0 = b * kNB;
-        hipLaunchKernelGGL(solve_diag_kernel<false>, dim3(gc), dim3(256), 0, s, B, ldb, n, nrhs, k0, Winv, info);
-        const int64_t m = n - k0 - kNB;
-        if (m > 0)
-            hipLaunchKernelGGL(solve_update_kernel<false>, dim3((unsigned)((m + 63) / 64), gc), dim3(256), 0, s, L,
-                               lda, n, B, ldb, nrhs, k0, info);
-    }
-    for (int64_t b = nblk - 1; b >= 0; --b) {  // backward: L^T X = Y
-        const int64_t k0 = b * kNB;
-        hipLaunchKernelGGL(solve_diag_kernel<true>, dim3(gc), dim3(256), 0, s, B, ldb, n, nrhs, k0, Winv, info);
-        if (k0 > 0)
-            hipLaunchKernelGGL(solve_update_kernel<true>, dim3((unsigned)((k0 + 63) / 64), gc), dim3(256), 0, s, L,
-                               lda, n, B, ldb, nrhs, k0, info);
-    }
+    launch_syrk(A + r0 + r0 * lda, lda, A + r0 + k0 * lda, lda, n - r0, kNB, info, s);
 }
 
 // ------------------------------------------------------------------------------ helpers

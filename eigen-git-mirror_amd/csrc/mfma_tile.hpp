@@ -1,0 +1,181 @@
+// mfma_tile.hpp -- FP64 MFMA wave-tile engine with operands read straight from global
+// memory (L2) into registers.
+//
+// An FP64 MFMA 16x16x4 occupies the matrix core for 64 cycles, so a wave tile of
+// TA x TB MFMA tiles consumes 4 k-values of (16*TA + 16*TB) rows every 64*TA*TB cycles:
+// a few bytes per cycle per CU, far below L2 bandwidth.  Staging through LDS would only add
+// barriers; instead each wave loads its own fragments and keeps D k-steps of loads in
+// flight in a rolling register buffer, so a workgroup's waves never synchronise.
+//
+// Operand convention (see mfma_f64 in common.hpp): C[i][j] += sum_k P[i][k] * Q[j][k],
+// both operands "rows x k", column-major: element (r, k) at X[r + k * ldx].  Rows past the
+// operand's end are clamped (their products land in C rows/cols the caller never stores);
+// k past kmax reads as 0.
+#pragma once
+#include "common.hpp"
+
+namespace mi {
+
+// MASKED: k may stop short of a multiple of 4 (kmax % 4 != 0); the mask is applied at
+// MFMA time so that loads stay in flight (a select right after a load would force a wait).
+template <int TA, int TB, int D = 3, bool MASKED = false>
+struct WaveTile {
+    static constexpr int U = D + 1;  // rolling buffer depth
+    dvec4 acc[TA][TB];
+    const double *pa[TA];
+    const double *qb[TB];
+    int64_t ldp, ldq;
+    int kmax;  // number of valid k
+
+    __device__ __forceinline__ void zero()
+    {
+#pragma unroll
+        for (int a = 0; a < TA; ++a)
+#pragma unroll
+            for (int b = 0; b < TB; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
+    }
+
+    // P rows p0 .. p0+16*TA-1 (valid rows < pmax), Q rows q0 .. (valid < qmax).
+    __device__ __forceinline__ void setup(const double *P, int64_t ldp_, int p0, int pmax, const double *Q,
+                                          int64_t ldq_, int q0, int qmax, int kmax_)
+    {
+        const int l = lane_id();
+        ldp = ldp_;
+        ldq = ldq_;
+        kmax = kmax_;
+#pragma unroll
+        for (int a = 0; a < TA; ++a) {
+            int r = p0 + 16 * a + (l & 15);
+            r = r < pmax ? r : pmax - 1;
+            pa[a] = P + r;
+        }
+#pragma unroll
+        for (int b = 0; b < TB; ++b) {
+            int r = q0 + 16 * b + (l & 15);
+            r = r < qmax ? r : qmax - 1;
+            qb[b] = Q + r;
+        }
+    }
+
+    struct Frag {
+        double p[TA], q[TB];
+    };
+
+    // Raw loads only, always from in-bounds addresses: the step is clamped to the last one
+    // (prefetches past the end re-read it) and, when MASKED, lanes whose k is past kmax read
+    // column 0.  mma() zeroes what must not count.
+    __device__ __forceinline__ void load(Frag &f, int s, int S) const
+    {
+        s = s < S ? s : S - 1;
+        int k = 4 * s + (lane_id() >> 4);
+        if (MASKED) k = k < kmax ? k : 0;
+        const int64_t op = (int64_t)k * ldp, oq = (int64_t)k * ldq;
+#pragma unroll
+        for (int a = 0; a < TA; ++a) f.p[a] = pa[a][op];
+#pragma unroll
+        for (int b = 0; b < TB; ++b) f.q[b] = qb[b][oq];
+    }
+
+    __device__ __forceinline__ void mma(const Frag &f, int s)
+    {
+        bool ok = true;
+        if (MASKED) ok = 4 * s + (lane_id() >> 4) < kmax;
+#pragma unroll
+        for (int a = 0; a < TA; ++a)
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+                const double q = MASKED ? (ok ? f.q[b] : 0.0) : f.q[b];
+                acc[a][b] = mfma_f64(q, f.p[a], acc[a][b]);
+            }
+    }
+
+    // Accumulate over k in [0, kmax), 4 k per MFMA step, D steps of loads in flight.  The
+    // loop body has no branches, so the compiler's wait counts stay exact: the step count
+    // is padded to a multiple of U, which requires MASKED unless kmax % (4U) == 0.
+    __device__ __forceinline__ void run()
+    {
+        const int S = (kmax + 3) >> 2;
+        const int Sp = (S + U - 1) / U * U;
+        Frag buf[U];
+#pragma unroll
+        for (int u = 0; u < D; ++u) load(buf[u], u, S);
+        for (int s0 = 0; s0 < Sp; s0 += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int s = s0 + u;
+                load(buf[(u + D) % U], s + D, S);
+                mma(buf[u], s);
+                // keep each step's loads where they are: hoisting them reorders the wait
+                // counts so a step ends up waiting for the newest prefetches
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+
+    // Same for a compile-time number of steps S (kmax == 4S), fully unrolled: without a loop
+    // back edge the wait counts are exact for every step.
+    template <int S>
+    __device__ __forceinline__ void run_fixed()
+    {
+        Frag buf[U];
+#pragma unroll
+        for (int u = 0; u < D; ++u) load(buf[u], u, S);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (s + D < S) load(buf[(s + D) % U], s + D, S);
+            mma(buf[s % U], s);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    // C[i][j] -= acc (or = acc when ASSIGN) for the elements with i < imax, j < jmax and,
+    // when LOWER, j <= i; i = row0 + row_of(a), j = col0 + col_of(b, r).  All loads of C are
+    // issued before any store, so they overlap.
+    template <bool ASSIGN, bool LOWER>
+    __device__ __forceinline__ void store(double *C, int64_t ldc, int row0, int col0, int imax, int jmax) const
+    {
+        double old[TA][TB][4];
+        if (!ASSIGN) {  // unconditional loads from clamped (in-bounds) addresses
+#pragma unroll
+            for (int a = 0; a < TA; ++a)
+#pragma unroll
+                for (int b = 0; b < TB; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        int i = row0 + row_of(a), j = col0 + col_of(b, r);
+                        i = i < imax ? i : imax - 1;
+                        j = j < jmax ? j : jmax - 1;
+                        old[a][b][r] = C[i + (int64_t)j * ldc];
+                    }
+        }
+#pragma unroll
+        for (int a = 0; a < TA; ++a)
+#pragma unroll
+            for (int b = 0; b < TB; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = row0 + row_of(a), j = col0 + col_of(b, r);
+                    const bool ok = i < imax && j < jmax && (!LOWER || j <= i);
+                    if (ok) C[i + (int64_t)j * ldc] = ASSIGN ? acc[a][b][r] : old[a][b][r] - acc[a][b][r];
+                }
+    }
+
+    // Element r (0..3) of tile (a, b) held by this lane is C[i][j] with
+    // i = row0 + 16a + (lane % 16), j = col0 + 16b + 4r + lane / 16.
+    __device__ __forceinline__ static int row_of(int a) { return 16 * a + (lane_id() & 15); }
+    __device__ __forceinline__ static int col_of(int b, int r) { return 16 * b + 4 * r + (lane_id() >> 4); }
+};
+
+// Workgroup -> tile remap that gives each XCD a contiguous range of the tile sequence
+// (workgroups are dispatched round-robin over the 8 XCDs; each XCD has its own L2).
+__device__ __forceinline__ int xcd_remap(int w, int ntiles)
+{
+    constexpr int X = 8;
+    if (ntiles < 2 * X) return w;
+    const int x = w % X, idx = w / X;
+    const int q = ntiles / X, rem = ntiles % X;
+    const int start = x * q + (x < rem ? x : rem);
+    return start + idx;
+}
+
+}  // namespace mi

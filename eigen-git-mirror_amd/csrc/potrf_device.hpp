@@ -1,0 +1,359 @@
+// potrf_device.hpp -- device-side pieces of the diagonal-block factor and the panel solve,
+// shared by the stand-alone kernels (potrf_diag.hip) and the fused factorization step
+// (factor_step.hip).  See potrf_diag.hip for the algorithms and their reference citations.
+#pragma once
+#include "common.hpp"
+#include "kernels.hpp"
+#include "mfma_tile.hpp"
+
+namespace mi {
+
+constexpr int LDL = 144;  // LDS leading dimension: rows 16 apart land in opposite bank halves
+
+__device__ __forceinline__ double readlane_d(double x, int lane)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Wave-level factor of the panel S[c0.., c0..c0+15] (rows c0..127): lane l holds rows
+// c0+l and c0+l+64 in v0/v1.  Returns the failing column (panel-local) or -1; on success
+// the panel is written back to S and v0 of lanes 0..15 keeps the leaf L_pp (row l).
+// Scaling by 1/sqrt(x) is a multiplication by the reciprocal of the stored diagonal.
+__device__ __forceinline__ int panel_factor(double *S, int c0, double (&v0)[16])
+{
+    const int l = lane_id();
+    const int r0 = c0 + l, r1 = c0 + l + 64;
+    double v1[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const double a0 = S[(r0 < 128 ? r0 : 127) + (c0 + c) * LDL];
+        const double a1 = S[(r1 < 128 ? r1 : 127) + (c0 + c) * LDL];
+        v0[c] = (r0 < 128 && (l >= 16 || c <= l)) ? a0 : 0.0;
+        v1[c] = (r1 < 128) ? a1 : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const double x = readlane_d(v0[j], j);  // pivot: row c0+j lives in lane j
+        if (x <= 0.0) return j;
+        // 1/sqrt(x) from the hardware estimate and one Newton step (relative error ~1 ulp),
+        // d = x * rd; the chain is what serialises the 128 columns of the block
+        double rd = __builtin_amdgcn_rsq(x);
+        rd = rd * (1.5 - 0.5 * x * rd * rd);
+        const double d = x * rd;
+        if (l == j) v0[j] = d;
+        if (l > j) v0[j] *= rd;
+        v1[j] *= rd;
+#pragma unroll
+        for (int c = j + 1; c < 16; ++c) {
+            const double lcj = readlane_d(v0[j], c);  // L(c0+c, c0+j)
+            v0[c] -= v0[j] * lcj;
+            v1[c] -= v1[j] * lcj;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        if (r0 < 128 && (l >= 16 || c <= l)) S[r0 + (c0 + c) * LDL] = v0[c];
+        if (r1 < 128) S[r1 + (c0 + c) * LDL] = v1[c];
+    }
+    return -1;
+}
+
+// X = L_pp^{-1} of the 16 x 16 leaf held by lanes 0..15 (row l of L_pp in v0), written to
+// xleaf (column-major, zeros above the diagonal).  Lane i builds row i of X.
+__device__ __forceinline__ void leaf_inverse(const double (&v0)[16], double *__restrict__ xleaf)
+{
+    const int l = lane_id();
+    double x[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) x[c] = (c == l) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const double rk = 1.0 / readlane_d(v0[k], k);
+        if (l == k) {
+#pragma unroll
+            for (int c = 0; c <= k; ++c) x[c] *= rk;
+        }
+#pragma unroll
+        for (int c = 0; c <= k; ++c) {
+            const double xkc = readlane_d(x[c], k);
+            if (l > k) x[c] -= v0[k] * xkc;
+        }
+    }
+    if (l < 16) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) xleaf[l + 16 * c] = (c <= l) ? x[c] : 0.0;
+    }
+}
+
+// LDS of one workgroup, shared by the two roles: the 128 x 128 block (ld LDL) for the
+// factor, or L11 (ld 129) plus the 8 leaf inverses for the panel solve.
+constexpr int kSmem = 128 * 129 + 8 * 256;
+static_assert(kSmem >= 128 * LDL, "LDS buffer too small for the block");
+
+// A (global, ld lda, valid order bs) -> S (lower part, identity beyond bs); 32 loads in
+// flight per thread, from clamped in-bounds addresses.
+__device__ __forceinline__ void block_to_lds(double *S, const double *__restrict__ Ab, int64_t lda, int bs)
+{
+    const int tid = threadIdx.x, r = tid & 127, ch = tid >> 7;
+#pragma unroll
+    for (int q0 = 0; q0 < 64; q0 += 32) {
+        double a[32];
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int c = 2 * (q0 + q) + ch;
+            const bool in = r < bs && c < bs;
+            a[q] = Ab[(in ? r : 0) + (int64_t)(in ? c : 0) * lda];
+        }
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int c = 2 * (q0 + q) + ch;
+            const bool in = r < bs && c < bs;
+            if (r >= c) S[r + c * LDL] = in ? a[q] : (r == c ? 1.0 : 0.0);
+        }
+    }
+}
+
+__device__ __forceinline__ void lds_to_block(const double *S, double *__restrict__ Ab, int64_t lda, int bs)
+{
+    const int tid = threadIdx.x, r = tid & 127, ch = tid >> 7;
+#pragma unroll 8
+    for (int q = 0; q < 64; ++q) {
+        const int c = 2 * q + ch;
+        if (r >= c && r < bs) Ab[r + (int64_t)c * lda] = S[r + c * LDL];
+    }
+}
+
+// Cholesky of the 128 x 128 lower block in S (all 256 threads).  Returns the first failing
+// column or -1 (uniform across the workgroup); writes the 8 leaf inverses to Xl.
+__device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl, int *s_fail)
+{
+    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    if (tid == 0) *s_fail = -1;
+    __syncthreads();
+    for (int p = 0; p < 8; ++p) {
+        const int c0 = 16 * p;
+        double v0[16];
+        if (w == 0) {
+            const int f = panel_factor(S, c0, v0);
+            if (f >= 0 && l == 0) *s_fail = c0 + f;
+        }
+        __syncthreads();
+        if (*s_fail >= 0) return *s_fail;
+        if (w == 0) {
+            leaf_inverse(v0, Xl + p * 256);
+        } else {
+            // rank-16 update of S[c0+16.., c0+16..] (lower) by the panel columns c0..c0+15:
+            // waves 1..3 take tiles w-1, w+2, ... (at most 10); all MFMAs first (the panel
+            // columns are not written here), then the read-modify-writes
+            const int T = 7 - p;
+            const int nt = T * (T + 1) / 2;
+            dvec4 acc[10];
+            int tr[10], tc[10];
+#pragma unroll
+            for (int h = 0; h < 10; ++h) {
+                const int t = (w - 1) + 3 * h;
+                int ti = 0, tj = 0;
+                if (t < nt) lower_tile(t, ti, tj);
+                tr[h] = c0 + 16 + 16 * ti;
+                tc[h] = c0 + 16 + 16 * tj;
+                acc[h] = dvec4{0.0, 0.0, 0.0, 0.0};
+                if (t < nt) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int kc = c0 + 4 * s + (l >> 4);
+                        acc[h] = mfma_f64(S[tc[h] + (l & 15) + kc * LDL], S[tr[h] + (l & 15) + kc * LDL], acc[h]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 10; ++h) {
+                if ((w - 1) + 3 * h < nt) {
+                    double old[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) old[r] = S[tr[h] + (l & 15) + (tc[h] + 4 * r + (l >> 4)) * LDL];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = tr[h] + (l & 15), j = tc[h] + 4 * r + (l >> 4);
+                        if (i >= j) S[i + j * LDL] = old[r] - acc[h][r];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    return -1;
+}
+
+// Rows 0..127 of P (valid m) <- P * L^{-T} with L (128 x 128 lower, ld ldl, valid order bs,
+// identity beyond) and its 8 leaf inverses Xl, using smem for L and the leaves.  Each wave
+// owns 2 row tiles of 16 rows.  IDENTITY: P is the identity (not read); the result L^{-T}
+// goes to O (ld 128) and its transpose L^{-1} to O + 128*128.
+template <bool IDENTITY>
+__device__ __forceinline__ void trsm_rows(double *smem, double *__restrict__ P, int64_t ldp, int m,
+                                          const double *__restrict__ L, int64_t ldl, int bs,
+                                          const double *__restrict__ Xl, double *__restrict__ O)
+{
+    const int l = lane_id(), w = threadIdx.x >> 6;
+    const int row0 = 32 * w;
+    double *Ls = smem;
+    double *Xs = smem + 128 * 129;
+    for (int e = threadIdx.x; e < 128 * 128; e += 256) {
+        const int r = e & 127, c = e >> 7;
+        const bool in = r < bs && c < bs;
+        const double v = L[(in ? r : 0) + (int64_t)(in ? c : 0) * ldl];
+        Ls[r + 129 * c] = (r >= c) ? (in ? v : (r == c ? 1.0 : 0.0)) : 0.0;
+    }
+    for (int e = threadIdx.x; e < 8 * 256; e += 256) Xs[e] = Xl[e];
+    __syncthreads();
+
+    dvec4 sol[2][8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        dvec4 acc[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int i = row0 + 16 * t + (l & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = 16 * p + 4 * r + (l >> 4);
+                if (IDENTITY) {
+                    acc[t][r] = (i == j) ? 1.0 : 0.0;
+                } else {
+                    const double pv = P[(i < m ? i : m - 1) + (int64_t)j * ldp];
+                    acc[t][r] = (i < m) ? pv : 0.0;
+                }
+            }
+        }
+        // acc -= sum_{q<p} sol_q * L[16p.., 16q..]^T
+#pragma unroll
+        for (int q = 0; q < p; ++q) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int li = 16 * p + (l & 15), lk = 16 * q + 4 * s + (l >> 4);
+                const double lv = Ls[li + 129 * lk];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) acc[t] = mfma_f64(-lv, sol[t][q][s], acc[t]);
+            }
+        }
+        // sol_p = acc * X_pp^T
+        dvec4 z[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const double xv = Xs[p * 256 + (l & 15) + 16 * (4 * s + (l >> 4))];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) z[t] = mfma_f64(xv, acc[t][s], z[t]);
+        }
+        sol[0][p] = z[0];
+        sol[1][p] = z[1];
+    }
+    double *Out = IDENTITY ? O : P;
+    const int64_t ldo = IDENTITY ? 128 : ldp;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int i = row0 + 16 * t + (l & 15);
+        if (IDENTITY || i < m) {
+#pragma unroll
+            for (int p = 0; p < 8; ++p)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = 16 * p + 4 * r + (l >> 4);
+                    Out[i + (int64_t)j * ldo] = sol[t][p][r];
+                    if (IDENTITY) Out[128 * 128 + j + 128 * i] = sol[t][p][r];
+                }
+        }
+    }
+}
+
+// The panel role of one block step (see panel_fused_kernel); g is the role from the ticket.
+// smem: kSmem doubles of LDS; s_fail/s_ok: LDS ints.
+__device__ __forceinline__ void panel_role(int g, double *smem, int *s_fail, int *s_ok, double *__restrict__ A,
+                                           int64_t lda, int64_t n, int64_t k0, double *__restrict__ Wleaf, int *info,
+                                           uint32_t *flag, uint32_t epoch, int *status)
+{
+    const int tid = threadIdx.x, w = tid >> 6;
+    const int64_t R0 = k0 + 128 * (int64_t)g;
+    const int rows = (int)((n - R0) < 128 ? (n - R0) : 128);
+    const int bs = (int)((n - k0) < 128 ? (n - k0) : 128);
+    double *Ck = A + R0 + k0 * lda;  // this role's 128 rows of block column k0
+    double *Xl = Wleaf + (size_t)(k0 / 128) * kLeafDoubles;
+
+    if (k0 > 0) {
+        // C -= P_g P_0^T, P = previous panel (columns k0-128..k0-1): rows R0.. and k0..
+        const int wr = w >> 1, wc = w & 1;
+        const bool active = !(g == 0 && wr == 0 && wc == 1);  // strictly upper quadrant of the diagonal tile
+        WaveTile<4, 4> t;
+        t.zero();
+        if (active) {
+            t.setup(A + R0 + (k0 - 128) * lda, lda, wr * 64, rows, A + k0 + (k0 - 128) * lda, lda, wc * 64, bs, 128);
+            t.run_fixed<32>();
+        }
+        if (g == 0) {
+            double *S = smem;
+            if (active) {  // S = A_kk - acc on the lower part, identity beyond bs
+                double old[4][4][4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            int i = wr * 64 + t.row_of(a), j = wc * 64 + t.col_of(b, r);
+                            i = i < bs ? i : bs - 1;
+                            j = j < bs ? j : bs - 1;
+                            old[a][b][r] = Ck[i + (int64_t)j * lda];
+                        }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int i = wr * 64 + t.row_of(a), j = wc * 64 + t.col_of(b, r);
+                            if (i >= j) {
+                                const bool in = i < bs && j < bs;
+                                S[i + j * LDL] = in ? old[a][b][r] - t.acc[a][b][r] : (i == j ? 1.0 : 0.0);
+                            }
+                        }
+            }
+        } else {
+            t.store<false, false>(Ck, lda, wr * 64, wc * 64, rows, 128);
+        }
+    } else if (g == 0) {
+        block_to_lds(smem, Ck, lda, bs);
+    }
+
+    if (g == 0) {
+        const int f = potrf_in_lds(smem, Xl, s_fail);  // begins with a barrier
+        if (f >= 0) {
+            if (tid == 0) atomicCAS(info, 0, (int)(k0 + f + 1));
+        } else {
+            lds_to_block(smem, Ck, lda, bs);
+        }
+        __syncthreads();  // every wave's stores of L11 and the leaves are complete
+        if (tid == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    __syncthreads();  // this role's updated rows are stored before they are read back below
+    if (tid == 0) {
+        uint32_t spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > (1u << 22)) {
+                atomicOr(status, MI_PANEL_TIMEOUT);
+                ok = 0;
+                break;
+            }
+        }
+        if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *s_ok = ok;
+    }
+    __syncthreads();
+    if (!*s_ok || factor_failed(info)) return;
+    trsm_rows<false>(smem, Ck, lda, rows, A + k0 + k0 * lda, lda, 128, Xl, nullptr);
+}
+
+}  // namespace mi

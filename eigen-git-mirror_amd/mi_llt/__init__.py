@@ -52,13 +52,19 @@ def lib() -> ctypes.CDLL:
         L.mi_dpotrs.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64]
         L.mi_dposv_async.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64, _VP]
         L.mi_dsy_l1norm.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, _VP]
+        L.mi_llt_trace.argtypes = [_VP, _INT]
+        L.mi_llt_trace_read.argtypes = [_VP, _INT, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]
+        L.mi_llt_trace_reset.argtypes = [_VP]
+        L.mi_llt_status.argtypes = [_VP, ctypes.POINTER(_INT)]
         L.LAPACKE_dpotrf.argtypes = [_INT, ctypes.c_char, _INT, _VP, _INT]
         L.LAPACKE_dpotrs.argtypes = [_INT, ctypes.c_char, _INT, _INT, _VP, _INT, _VP, _INT]
         L.dpotrf_.argtypes = [ctypes.c_char_p, ctypes.POINTER(_INT), _VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]
         L.dpotrs_.argtypes = [ctypes.c_char_p, ctypes.POINTER(_INT), ctypes.POINTER(_INT), _VP,
                               ctypes.POINTER(_INT), _VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]
         for name in ("mi_llt_create", "mi_llt_destroy", "mi_llt_set_stream", "mi_dpotrf", "mi_dpotrf_async",
-                     "mi_dpotrs", "mi_dposv_async", "mi_dsy_l1norm", "LAPACKE_dpotrf", "LAPACKE_dpotrs"):
+                     "mi_dpotrs", "mi_dposv_async", "mi_dsy_l1norm", "LAPACKE_dpotrf", "LAPACKE_dpotrs",
+                     "mi_llt_trace", "mi_llt_trace_read", "mi_llt_trace_reset", "mi_llt_status"):
             getattr(L, name).restype = _INT
         L.dpotrf_.restype = None
         L.dpotrs_.restype = None
@@ -112,6 +118,30 @@ class Handle:
 
     def workspace_bytes(self) -> int:
         return int(lib().mi_llt_workspace_bytes(self._h))
+
+    def status(self) -> int:
+        """Waits for the stream; device status bits since the last call (0 = all good)."""
+        v = ctypes.c_int(0)
+        self.check(lib().mi_llt_status(self._h, ctypes.byref(v)), "mi_llt_status")
+        return int(v.value)
+
+    TRACE_KINDS = {"factor_step": 0, "trsm": 1, "syrk": 2, "solve": 3, "trtri": 4, "panel_update": 5}
+
+    def trace(self, enable: bool = True):
+        self.check(lib().mi_llt_trace(self._h, int(enable)), "mi_llt_trace")
+
+    def trace_reset(self):
+        self.check(lib().mi_llt_trace_reset(self._h), "mi_llt_trace_reset")
+
+    def trace_read(self) -> dict:
+        """{kind: (total_ms, algorithmic_flops, launches)} since the last reset."""
+        out = {}
+        for name, k in self.TRACE_KINDS.items():
+            ms, fl, nl = ctypes.c_double(), ctypes.c_double(), _I64()
+            self.check(lib().mi_llt_trace_read(self._h, k, ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(nl)),
+                       "mi_llt_trace_read")
+            out[name] = (ms.value, fl.value, int(nl.value))
+        return out
 
 
 _default: Handle | None = None

@@ -55,6 +55,29 @@ int mi_llt_set_stream(mi_llt_handle h, void *stream);
 const char *mi_llt_last_error(mi_llt_handle h);
 /* Device bytes currently held by the handle's workspace. */
 int64_t mi_llt_workspace_bytes(mi_llt_handle h);
+/* Waits for the handle's stream, then returns (and clears) the device status bits of the
+ * asynchronous calls since the last query: 0, or a combination of MI_LLT_STATUS_SOLVE_TIMEOUT
+ * (a solve sweep gave up waiting for a block) and MI_LLT_STATUS_PANEL_TIMEOUT (a panel solve
+ * gave up waiting for its diagonal block); the affected result is then invalid. */
+#define MI_LLT_STATUS_SOLVE_TIMEOUT 1
+#define MI_LLT_STATUS_PANEL_TIMEOUT 2
+int mi_llt_status(mi_llt_handle h, int *status);
+
+/* ---- measurement (no reference counterpart: Eigen times with bench/BenchTimer.h) -------
+ * When tracing is on, each launch of the kinds below is bracketed by HIP events on the
+ * handle's stream; mi_llt_trace_read waits for the stream and returns, per kind, the summed
+ * event time (ms), the summed algorithmic flops (LAPACK counts) and the launch count since
+ * the last reset.  MI_TRACE_SOLVE brackets a whole forward+backward solve. */
+#define MI_TRACE_POTRF 0   /* factorization step: panel (update, POTRF, TRSM) + trailing SYRK */
+#define MI_TRACE_TRSM 1    /* panel A21 <- A21 L11^{-T} */
+#define MI_TRACE_SYRK 2    /* trailing A22 -= A21 A21^T */
+#define MI_TRACE_SOLVE 3   /* forward + backward substitution */
+#define MI_TRACE_TRTRI 4   /* diagonal-block inverses L_bb^{-T} built for the solve */
+#define MI_TRACE_GEMM 5    /* lookahead: update of the next panel by the current one */
+#define MI_TRACE_KINDS 8
+int mi_llt_trace(mi_llt_handle h, int enable);
+int mi_llt_trace_read(mi_llt_handle h, int kind, double *ms, double *flops, int64_t *launches);
+int mi_llt_trace_reset(mi_llt_handle h);
 
 /* ---- factor ---------------------------------------------------------------------------
  * In-place Cholesky of the `uplo` triangle ('L' or 'U') of the n x n column-major matrix

@@ -31,6 +31,10 @@ def to_dev(x, dev):
     return mi_llt.colmajor(torch.from_numpy(np.asarray(x, dtype=np.float64)).to(dev))
 
 
+def assert_status_ok():
+    assert mi_llt.default_handle().status() == 0, "a solve sweep reported a timeout"
+
+
 def gpu_factor(a, dev, uplo="L"):
     da = to_dev(a, dev)
     info = mi_llt.potrf_(da, uplo)
@@ -60,6 +64,7 @@ def test_factor_solve_matches_eigen_golden(dev, tag):
     d_info = torch.zeros(1, dtype=torch.int32, device=dev)
     mi_llt.posv_async_(da, db, d_info)
     torch.cuda.synchronize()
+    assert_status_ok()
     assert int(d_info.item()) == 0
     m, x = da.cpu().numpy(), db.cpu().numpy()
     np.testing.assert_array_equal(np.triu(m, 1), np.triu(g[f"{tag}/LLT"], 1))
@@ -119,6 +124,7 @@ def test_solve_matches_oracle(dev, n, r):
     db = to_dev(b, dev)
     mi_llt.potrs_(da, db)
     torch.cuda.synchronize()
+    assert_status_ok()
     x = db.cpu().numpy()
     xo = oracle.llt_solve(np.tril(oracle.llt_blocked(a)[0]), b)
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-10 * max(1.0, np.abs(xo).max()))
@@ -224,6 +230,7 @@ def test_full_size_backward_error(dev, n, r):
     d_info = torch.zeros(1, dtype=torch.int32, device=dev)
     mi_llt.posv_async_(da, db, d_info)
     torch.cuda.synchronize()
+    assert_status_ok()
     assert int(d_info.item()) == 0
     l = torch.tril(da)
     asym = torch.tril(a) + torch.tril(a, -1).t()
