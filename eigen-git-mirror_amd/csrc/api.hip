@@ -21,8 +21,6 @@
 
 struct mi_llt_context {
     hipStream_t stream = nullptr;       // caller's stream: all work is ordered on it
-    hipStream_t panel = nullptr;        // internal stream for the lookahead panel work
-    std::vector<hipEvent_t> events;     // per-step dependency events (grow-only pool)
     int device = 0;
     void *ws = nullptr;       // grow-only device workspace
     size_t ws_bytes = 0;
@@ -82,16 +80,6 @@ void traced(mi_llt_handle h, hipStream_t s, int kind, double flops, F &&launch)
     launch();
     hipEventRecord(r.e1, s);
     h->recs.push_back(r);
-}
-
-int ensure_events(mi_llt_handle h, size_t count)
-{
-    while (h->events.size() < count) {
-        hipEvent_t e;
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return MI_LLT_E_HIP;
-        h->events.push_back(e);
-    }
-    return 0;
 }
 
 size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -182,7 +170,8 @@ double panel_flops(int64_t n, int64_t k0, int64_t NB)
     return (k0 > 0 ? 2.0 * mc * b * NB - b * (b - 1) * NB : 0.0) + b * b * b / 3 + b * b / 2 + b / 6 + m * b * b;
 }
 
-int factor_lower(mi_llt_handle h, double *A, int64_t lda, int64_t n, double *wleaf, int *d_info)
+template <typename T>
+int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, int *d_info, int prec = MI_PREC_FP64)
 {
     constexpr int64_t NB = mi::kNB;
     const int64_t nblk = (n + NB - 1) / NB;
@@ -203,16 +192,27 @@ int factor_lower(mi_llt_handle h, double *A, int64_t lda, int64_t n, double *wle
         const uint32_t epoch = (kp >= 0) ? ++h->panel_epoch : h->panel_epoch;
         uint32_t grid = 0;
         traced(h, s, MI_TRACE_POTRF, fl, [&] {
-            mi::launch_factor_step(A, lda, n, kp, ks, wleaf, d_info, h->sync + 4, h->panel_tickets, h->sync + 5, epoch,
-                                   (int *)h->sync, s, &grid);
+            mi::launch_factor_step<T>(A, lda, n, kp, ks, wleaf, d_info, h->sync + 4, h->panel_tickets, h->sync + 5,
+                                      epoch, (int *)h->sync, s, &grid, prec);
         });
         h->panel_tickets += grid;
     }
     return 0;
 }
 
-int solve_lower(mi_llt_handle h, const double *L, int64_t ldl, int64_t n, double *B, int64_t ldb, int64_t nrhs,
-                const double *wleaf, double *winv, const int *d_info)
+// Diagonal-block inverses of a factor (needs its leaf inverses), for solve_only.
+template <typename T>
+void block_inverses(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, const double *wleaf, double *winv,
+                    const int *d_info)
+{
+    traced(h, h->stream, MI_TRACE_TRTRI, (double)n * mi::kNB * mi::kNB / 3.0,
+           [&] { mi::launch_block_inverses<T>(L, ldl, n, wleaf, winv, d_info, h->stream); });
+}
+
+// Forward + backward substitution in place on B with a factor whose block inverses are in winv.
+template <typename T>
+int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, int64_t ldb, int64_t nrhs,
+               const double *winv, const int *d_info)
 {
     const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
     const int64_t ngrp = (nrhs + 15) / 16;
@@ -224,13 +224,75 @@ int solve_lower(mi_llt_handle h, const double *L, int64_t ldl, int64_t n, double
         h->epoch = 0;
     }
     const uint32_t ef = ++h->epoch, eb = ++h->epoch;
-    traced(h, h->stream, MI_TRACE_TRTRI, (double)n * mi::kNB * mi::kNB / 3.0,
-           [&] { mi::launch_block_inverses(L, ldl, n, wleaf, winv, d_info, h->stream); });
     traced(h, h->stream, MI_TRACE_SOLVE, 2.0 * (double)n * (double)n * (double)nrhs, [&] {
-        mi::launch_solve_persistent(L, ldl, n, B, ldb, nrhs, winv, h->sync + kSyncHeader, (int *)h->sync + 1, ef,
-                                    eb, d_info, (int *)h->sync, h->stream);
+        mi::launch_solve_persistent<T>(L, ldl, n, B, ldb, nrhs, winv, h->sync + kSyncHeader, (int *)h->sync + 1, ef,
+                                       eb, d_info, (int *)h->sync, h->stream);
     });
     return 0;
+}
+
+template <typename T>
+int solve_lower(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, int64_t ldb, int64_t nrhs,
+                const double *wleaf, double *winv, const int *d_info)
+{
+    block_inverses<T>(h, L, ldl, n, wleaf, winv, d_info);
+    return solve_only<T>(h, L, ldl, n, B, ldb, nrhs, winv, d_info);
+}
+
+// Workspace of mi_dposv_mixed: [ W (FP32, or FP64 for the plain path) | Wleaf | Winv | R | D |
+// residual partials | norm partials | scalars: anorm, berr, rr[nrhs], xx[nrhs] ]
+struct MixedLayout {
+    float *w32 = nullptr;
+    double *w64 = nullptr;
+    int64_t ldw = 0;
+    double *wleaf = nullptr, *winv = nullptr, *R = nullptr, *D = nullptr, *part = nullptr, *tpart = nullptr;
+    double *anorm = nullptr, *berr = nullptr, *rr = nullptr, *xx = nullptr;
+};
+
+int prepare_mixed(mi_llt_handle h, int64_t n, int64_t nrhs, bool fp64, MixedLayout &m)
+{
+    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
+    const size_t wbytes = round_up((size_t)n * (size_t)n * (fp64 ? sizeof(double) : sizeof(float)));
+    const size_t leaf = round_up((size_t)nblk * mi::kLeafDoubles * sizeof(double));
+    const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
+    const size_t vec = round_up((size_t)n * (size_t)nrhs * sizeof(double));
+    const size_t part = round_up((size_t)mi::residual_chunks(n, nrhs) * (size_t)n * (size_t)nrhs * sizeof(double));
+    const size_t tpart = round_up((size_t)mi::sym_tiles64(n) * sizeof(double));
+    const size_t scal = round_up((size_t)(2 + 2 * nrhs) * sizeof(double));
+    int rc = reserve(h, wbytes + leaf + inv + 2 * vec + part + tpart + scal);
+    if (rc) return rc;
+    char *p = (char *)h->ws;
+    m = MixedLayout();
+    if (fp64)
+        m.w64 = (double *)p;
+    else
+        m.w32 = (float *)p;
+    m.ldw = n;
+    p += wbytes;
+    m.wleaf = (double *)p;
+    p += leaf;
+    m.winv = (double *)p;
+    p += inv;
+    m.R = (double *)p;
+    p += vec;
+    m.D = (double *)p;
+    p += vec;
+    m.part = (double *)p;
+    p += part;
+    m.tpart = (double *)p;
+    p += tpart;
+    m.anorm = (double *)p;
+    m.berr = m.anorm + 1;
+    m.rr = m.anorm + 2;
+    m.xx = m.rr + nrhs;
+    return 0;
+}
+
+int read_back(mi_llt_handle h, void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? 0 : fail_hip(h, e, "readback");
 }
 
 }  // namespace
@@ -247,12 +309,6 @@ int mi_llt_create(mi_llt_handle *h)
         delete c;
         return MI_LLT_E_ALLOC;
     }
-    // default priority (no stream priorities anywhere)
-    if (hipStreamCreateWithFlags(&c->panel, hipStreamNonBlocking) != hipSuccess) {
-        hipFree(c->d_info);
-        delete c;
-        return MI_LLT_E_HIP;
-    }
     *h = c;
     return 0;
 }
@@ -261,10 +317,7 @@ int mi_llt_destroy(mi_llt_handle h)
 {
     if (!valid(h)) return MI_LLT_E_HANDLE;
     hipStreamSynchronize(h->stream);
-    hipStreamSynchronize(h->panel);
-    for (hipEvent_t e : h->events) hipEventDestroy(e);
     if (h->sync) hipFree(h->sync);
-    hipStreamDestroy(h->panel);
     if (h->ws) hipFree(h->ws);
     if (h->d_info) hipFree(h->d_info);
     h->magic = 0;
@@ -401,7 +454,7 @@ int mi_dpotrs(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double 
         L = lay.tri;
         ldl = n;
     }
-    mi::launch_leaf_inverses(L, ldl, n, lay.wleaf, s);
+    mi::launch_leaf_inverses<double>(L, ldl, n, lay.wleaf, s);
     rc = solve_lower(h, L, ldl, n, dB, ldb, nrhs, lay.wleaf, lay.winv, nullptr);
     if (rc) return rc;
     return check_launch(h, "solve");
@@ -461,6 +514,98 @@ int mi_dsy_l1norm(mi_llt_handle h, char uplo, int64_t n, const double *dA, int64
     if (rc) return rc;
     mi::launch_sy_l1norm(dA, lda, n, up == 1, (double *)h->ws, d_norm, h->stream);
     return check_launch(h, "l1norm");
+}
+
+int mi_dposv_mixed(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double *dA, int64_t lda,
+                   const double *dB, int64_t ldb, double *dX, int64_t ldx, int prec, double tol, int maxit,
+                   int *iters, double *berr, int *info)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (dA == nullptr && n > 0) return -5;
+    if (lda < (n > 1 ? n : 1)) return -6;
+    if (dB == nullptr && n > 0 && nrhs > 0) return -7;
+    if (ldb < (n > 1 ? n : 1)) return -8;
+    if (dX == nullptr && n > 0 && nrhs > 0) return -9;
+    if (ldx < (n > 1 ? n : 1)) return -10;
+    if (prec < MI_PREC_FP64 || prec > MI_PREC_BF16X3) return -11;
+    if (!(tol >= 0.0)) return -12;
+    if (maxit < 0) return -13;
+    if (iters == nullptr) return -14;
+    if (berr == nullptr) return -15;
+    if (info == nullptr) return -16;
+    *iters = 0;
+    *berr = 0.0;
+    *info = 0;
+    if (n == 0 || nrhs == 0) return 0;
+    hipStream_t s = h->stream;
+    const bool upper = up == 1;
+    MixedLayout m;
+    int rc;
+
+    auto copy_b_to_x = [&]() -> int {
+        hipError_t e = hipMemcpy2DAsync(dX, (size_t)ldx * sizeof(double), dB, (size_t)ldb * sizeof(double),
+                                        (size_t)n * sizeof(double), (size_t)nrhs, hipMemcpyDeviceToDevice, s);
+        return e == hipSuccess ? 0 : fail_hip(h, e, "hipMemcpy2DAsync");
+    };
+    auto residual = [&]() {
+        traced(h, s, MI_TRACE_GEMM, 2.0 * (double)n * (double)n * (double)nrhs, [&] {
+            mi::launch_residual(dA, lda, n, upper, dX, ldx, dB, ldb, nrhs, m.R, n, m.part, m.rr, m.xx, m.anorm,
+                                m.berr, s);
+        });
+    };
+    // the plain FP64 factorization of a copy of A; `code` goes to *iters
+    auto fp64_path = [&](int code) -> int {
+        int r = prepare_mixed(h, n, nrhs, true, m);
+        if (r) return r;
+        hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, h->d_info, 0);
+        mi::launch_sym_copy_lower<double>(dA, lda, n, upper, m.w64, m.ldw, m.tpart, m.anorm, s);
+        r = factor_lower<double>(h, m.w64, m.ldw, n, m.wleaf, h->d_info);
+        if (r) return r;
+        int inf = 0;
+        if ((r = read_back(h, &inf, h->d_info, sizeof(int)))) return r;
+        *iters = code;
+        if (inf != 0) {
+            *info = inf;
+            return 0;
+        }
+        if ((r = copy_b_to_x())) return r;
+        if ((r = solve_lower<double>(h, m.w64, m.ldw, n, dX, ldx, nrhs, m.wleaf, m.winv, h->d_info))) return r;
+        residual();
+        if ((r = read_back(h, berr, m.berr, sizeof(double)))) return r;
+        return check_launch(h, "posv_mixed fp64");
+    };
+    if (prec == MI_PREC_FP64) return fp64_path(0);
+
+    if ((rc = prepare_mixed(h, n, nrhs, false, m))) return rc;
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, h->d_info, 0);
+    mi::launch_sym_copy_lower<float>(dA, lda, n, upper, m.w32, m.ldw, m.tpart, m.anorm, s);
+    if ((rc = factor_lower<float>(h, m.w32, m.ldw, n, m.wleaf, h->d_info, prec))) return rc;
+    int inf = 0;
+    if ((rc = read_back(h, &inf, h->d_info, sizeof(int)))) return rc;
+    if (inf != 0) return fp64_path(-1);  // not positive definite in the low-precision factor
+    block_inverses<float>(h, m.w32, m.ldw, n, m.wleaf, m.winv, h->d_info);
+    if ((rc = copy_b_to_x())) return rc;
+    if ((rc = solve_only<float>(h, m.w32, m.ldw, n, dX, ldx, nrhs, m.winv, h->d_info))) return rc;
+    for (int it = 0;; ++it) {
+        residual();
+        double b = 0.0;
+        if ((rc = read_back(h, &b, m.berr, sizeof(double)))) return rc;
+        *berr = b;
+        if (b <= tol) {
+            *iters = it;
+            return check_launch(h, "posv_mixed");
+        }
+        if (it == maxit) break;
+        hipError_t e = hipMemcpyAsync(m.D, m.R, (size_t)n * (size_t)nrhs * sizeof(double), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return fail_hip(h, e, "hipMemcpyAsync");
+        if ((rc = solve_only<float>(h, m.w32, m.ldw, n, m.D, n, nrhs, m.winv, h->d_info))) return rc;
+        mi::launch_add_in_place(dX, ldx, m.D, n, n, nrhs, s);
+    }
+    return fp64_path(-2);  // refinement did not reach tol
 }
 
 }  // extern "C"

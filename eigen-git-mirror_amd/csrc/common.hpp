@@ -4,6 +4,11 @@
 #include <stdint.h>
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Arithmetic of the trailing updates (the values of mi_prec in mi_llt.h).
+enum : int { kPrecF64 = 0, kPrecF32 = 1, kPrecBF16 = 2, kPrecBF16x3 = 3 };
 
 // FP64 MFMA 16x16x4.  Lane layout of v_mfma_f64_16x16x4_f64, measured on gfx950:
 //   operand a: lane l supplies a[i = l%16][k = l/16]
@@ -16,6 +21,21 @@ typedef double dvec4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ dvec4 mfma_f64(double q, double p, dvec4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(q, p, c, 0, 0, 0);
+}
+
+// FP32 MFMA 16x16x4 and BF16 MFMA 16x16x32 (FP32 accumulation).  Lane layout on gfx950:
+//   operand a: lane l supplies a[i = l%16][k = E*(l/16) + e], e < E (E = 1 for FP32, 8 for BF16)
+//   operand b: lane l supplies b[k = E*(l/16) + e][j = l%16]
+//   result   : lane l, element r holds d[i = 4*(l/16) + r][j = l%16]
+// With Q passed as `a` and P as `b` (C[i][j] += sum_k P[i][k] Q[j][k]), lane l, element r of
+// the result is C[i0 + l%16][j0 + 4*(l/16) + r].
+__device__ __forceinline__ fvec4 mfma_f32(float q, float p, fvec4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(q, p, c, 0, 0, 0);
+}
+__device__ __forceinline__ fvec4 mfma_bf16(bf16x8 q, bf16x8 p, fvec4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(q, p, c, 0, 0, 0);
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }

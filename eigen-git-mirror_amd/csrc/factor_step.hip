@@ -18,7 +18,10 @@ namespace mi {
 // kp: block column (first column) of the panel to factor, or -1 when none; Gp its roles.
 // ks: first column of the panel whose trailing update runs here, or -1 when none; the
 //     update covers rows/columns from r2 = ks + 256 on, T x T tiles (lower triangle).
-__global__ __launch_bounds__(256) void factor_step_kernel(double *__restrict__ A, int64_t lda, int64_t n, int64_t kp,
+// S: storage type (double, or float for the mixed-precision factorization), PREC: the
+// trailing-update arithmetic (kPrecF64 with double; FP32/BF16/split BF16 MFMA with float).
+template <typename S, int PREC>
+__global__ __launch_bounds__(256) void factor_step_kernel(S *__restrict__ A, int64_t lda, int64_t n, int64_t kp,
                                                           int Gp, int64_t ks, int T, double *__restrict__ Wleaf,
                                                           int *info, uint32_t *tickets, uint32_t ticket_base,
                                                           uint32_t *flag, uint32_t epoch, int *status)
@@ -32,19 +35,21 @@ __global__ __launch_bounds__(256) void factor_step_kernel(double *__restrict__ A
     const int t = s_role;
     const int nsyrk = T * (T + 1) / 2;
     if (Gp > 0 && (t == 0 || t > nsyrk)) {  // ticket 0: diagonal role; last Gp-1: row roles
-        panel_role(t == 0 ? 0 : t - nsyrk, smem, &s_fail, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch, status);
+        panel_role<S, PREC>(t == 0 ? 0 : t - nsyrk, smem, &s_fail, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch,
+                            status);
         return;
     }
     const int u = Gp > 0 ? t - 1 : t;
     const int64_t r2 = ks + 256;
     int ti, tj;
     lower_tile(xcd_remap(u, nsyrk), ti, tj);
-    syrk_tile<kNB>(A + r2 + r2 * lda, lda, A + r2 + ks * lda, lda, (int)(n - r2), (int)(n - r2), ti, tj);
+    syrk_tile<kNB, S, PREC>(A + r2 + r2 * lda, lda, A + r2 + ks * lda, lda, (int)(n - r2), (int)(n - r2), ti, tj);
 }
 
-void launch_factor_step(double *A, int64_t lda, int64_t n, int64_t kp, int64_t ks, double *Wleaf, int *info,
+template <typename S>
+void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t ks, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
-                        hipStream_t s, uint32_t *grid_out)
+                        hipStream_t s, uint32_t *grid_out, int prec)
 {
     const int Gp = kp >= 0 ? (int)((n - kp + 127) / 128) : 0;
     const int64_t m2 = ks >= 0 ? n - ks - 256 : 0;
@@ -52,8 +57,25 @@ void launch_factor_step(double *A, int64_t lda, int64_t n, int64_t kp, int64_t k
     const unsigned grid = (unsigned)(Gp + T * (T + 1) / 2);
     *grid_out = grid;
     if (grid == 0) return;
-    hipLaunchKernelGGL(factor_step_kernel, dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, m2 > 0 ? ks : (int64_t)-1,
-                       T, Wleaf, info, tickets, ticket_base, flag, epoch, status);
+    const int64_t ksx = m2 > 0 ? ks : (int64_t)-1;
+    if constexpr (sizeof(S) == 8) {
+        hipLaunchKernelGGL((factor_step_kernel<S, kPrecF64>), dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, ksx, T,
+                           Wleaf, info, tickets, ticket_base, flag, epoch, status);
+    } else if (prec == kPrecBF16) {
+        hipLaunchKernelGGL((factor_step_kernel<S, kPrecBF16>), dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, ksx, T,
+                           Wleaf, info, tickets, ticket_base, flag, epoch, status);
+    } else if (prec == kPrecBF16x3) {
+        hipLaunchKernelGGL((factor_step_kernel<S, kPrecBF16x3>), dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, ksx,
+                           T, Wleaf, info, tickets, ticket_base, flag, epoch, status);
+    } else {
+        hipLaunchKernelGGL((factor_step_kernel<S, kPrecF32>), dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, ksx, T,
+                           Wleaf, info, tickets, ticket_base, flag, epoch, status);
+    }
 }
+
+template void launch_factor_step<double>(double *, int64_t, int64_t, int64_t, int64_t, double *, int *, uint32_t *,
+                                         uint32_t, uint32_t *, uint32_t, int *, hipStream_t, uint32_t *, int);
+template void launch_factor_step<float>(float *, int64_t, int64_t, int64_t, int64_t, double *, int *, uint32_t *,
+                                        uint32_t, uint32_t *, uint32_t, int *, hipStream_t, uint32_t *, int);
 
 }  // namespace mi

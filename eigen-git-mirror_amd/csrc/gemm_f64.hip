@@ -1,70 +1,9 @@
-// gemm_f64.hip -- FP64 MFMA tile kernels of the blocked factorization and of the solve.
-//
-//   syrk_lower_kernel  A22 (lower) -= A21 * A21^T
-//       replaces the trailing rank update of llt_inplace::blocked, LLT.h:365
-//       (SelfadjointProduct.h:82-117 -> GeneralMatrixMatrixTriangular.h:57-194, whose cycles
-//       are spent in gebp_kernel, GeneralBlockPanelKernel.h:1548-2217).
-//
-// Products have the form C[i][j] -= sum_k P[i][k] * Q[j][k] on v_mfma_f64_16x16x4_f64, with
-// the operands read by each wave straight from L2 (mfma_tile.hpp).
+// gemm_f64.hip -- symmetric-storage helpers: triangle transposes for uplo = 'U' and the
+// symmetric L1 norm of LLT::compute.
 #include "common.hpp"
 #include "kernels.hpp"
-#include "syrk_device.hpp"
 
 namespace mi {
-
-// ------------------------------------------------------------------------------ SYRK
-// C (m x m, lower triangle) -= P P^T, P = m x K, one 128 x 128 tile per workgroup
-// (syrk_device.hpp).  FIRST_COL restricts the update to tile column 0 and to columns
-// j < ncol (the next panel's columns).
-template <bool FIRST_COL, int K>
-__global__ __launch_bounds__(256) void syrk_tiles_kernel(double *__restrict__ C, int64_t ldc,
-                                                         const double *__restrict__ P, int64_t ldp, int m,
-                                                         int ncol, const int *info)
-{
-    if (factor_failed(info)) return;
-    int ti, tj;
-    if (FIRST_COL) {
-        ti = blockIdx.x;
-        tj = 0;
-    } else {
-        const int T = (m + 127) / 128;
-        lower_tile(xcd_remap((int)blockIdx.x, T * (T + 1) / 2), ti, tj);
-    }
-    syrk_tile<K>(C, ldc, P, ldp, m, ncol, ti, tj);
-}
-
-// K is the depth of the update: 128 (one panel) or 256 (two panels).
-void launch_syrk(double *C, int64_t ldc, const double *P, int64_t ldp, int64_t m, int64_t K, const int *info,
-                 hipStream_t s)
-{
-    if (m <= 0) return;
-    const int T = (int)((m + 127) / 128);
-    const dim3 g(T * (T + 1) / 2);
-    if (K == 256)
-        hipLaunchKernelGGL((syrk_tiles_kernel<false, 256>), g, dim3(256), 0, s, C, ldc, P, ldp, (int)m, (int)m, info);
-    else
-        hipLaunchKernelGGL((syrk_tiles_kernel<false, 128>), g, dim3(256), 0, s, C, ldc, P, ldp, (int)m, (int)m, info);
-}
-
-void launch_syrk_first_col(double *C, int64_t ldc, const double *P, int64_t ldp, int64_t m, int64_t ncol,
-                           int64_t K, const int *info, hipStream_t s)
-{
-    if (m <= 0 || ncol <= 0) return;
-    const int T = (int)((m + 127) / 128);
-    if (K == 256)
-        hipLaunchKernelGGL((syrk_tiles_kernel<true, 256>), dim3(T), dim3(256), 0, s, C, ldc, P, ldp, (int)m,
-                           (int)ncol, info);
-    else
-        hipLaunchKernelGGL((syrk_tiles_kernel<true, 128>), dim3(T), dim3(256), 0, s, C, ldc, P, ldp, (int)m,
-                           (int)ncol, info);
-}
-
-void launch_syrk_update(double *A, int64_t lda, int64_t n, int64_t k0, const int *info, hipStream_t s)
-{
-    const int64_t r0 = k0 + kNB;
-    launch_syrk(A + r0 + r0 * lda, lda, A + r0 + k0 * lda, lda, n - r0, kNB, info, s);
-}
 
 // ------------------------------------------------------------------------------ helpers
 // W (lower) = A (upper)^T through 32x32 LDS tiles; tiles strictly above the diagonal of W

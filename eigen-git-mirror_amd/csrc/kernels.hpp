@@ -19,37 +19,27 @@ constexpr int64_t kInvDoubles = 2 * 128 * 128;  // X_b^T then X_b, each 128 x 12
 constexpr int MI_SOLVE_TIMEOUT = 1;
 constexpr int MI_PANEL_TIMEOUT = 2;
 
-// One fused block step (update of block column k0 by the previous panel, diagonal factor,
-// panel solve), ceil((n-k0)/128) workgroups; tickets/flag: persistent words of the handle,
-// ticket_base = number of tickets drawn before this launch, epoch = this step's flag value.
-void launch_panel_fused(double *A, int64_t lda, int64_t n, int64_t k0, double *Wleaf, int *info, uint32_t *tickets,
-                        uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status, hipStream_t s);
 // One factorization step: panel of block column kp (or none, kp < 0) + trailing update by
 // the panel at column ks of the columns from ks+256 on (or none, ks < 0).  Writes the grid
-// size (= tickets drawn) to *grid_out.
-void launch_factor_step(double *A, int64_t lda, int64_t n, int64_t kp, int64_t ks, double *Wleaf, int *info,
+// size (= tickets drawn) to *grid_out.  tickets/flag: persistent words of the handle,
+// ticket_base = number of tickets drawn before this launch, epoch = this step's flag value.
+// S = double: FP64 throughout (prec ignored); S = float: FP32 storage, FP64 diagonal factor
+// and panel solve, trailing updates in the MFMA arithmetic `prec` (kPrecF32/BF16/BF16x3).
+template <typename S>
+void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t ks, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
-                        hipStream_t s, uint32_t *grid_out);
-// Diagonal block at column k0: POTRF in place (lower) + its leaf inverses.
-void launch_potrf_diag(double *A, int64_t lda, int64_t n, int64_t k0, double *Wleaf, int *info, hipStream_t s);
-// Panel: A21 <- A21 * L11^{-T}, rows k0+128..n-1, columns k0..k0+127.
-void launch_trsm_panel(double *A, int64_t lda, int64_t n, int64_t k0, const double *Wleaf, const int *info,
-                       hipStream_t s);
+                        hipStream_t s, uint32_t *grid_out, int prec);
 // Winv[b] = {X_b^T, X_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).
-void launch_block_inverses(const double *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv,
-                           const int *info, hipStream_t s);
+template <typename T>
+void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv, const int *info,
+                           hipStream_t s);
 // Wleaf from an already factored matrix (standalone solve).
-void launch_leaf_inverses(const double *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s);
-// C (m x m lower) -= P P^T with P m x K; and its restriction to the first ncol columns.
-void launch_syrk(double *C, int64_t ldc, const double *P, int64_t ldp, int64_t m, int64_t K, const int *info,
-                 hipStream_t s);
-void launch_syrk_first_col(double *C, int64_t ldc, const double *P, int64_t ldp, int64_t m, int64_t ncol,
-                           int64_t K, const int *info, hipStream_t s);
-// Trailing update: A22 (lower) -= A21 * A21^T.
-void launch_syrk_update(double *A, int64_t lda, int64_t n, int64_t k0, const int *info, hipStream_t s);
+template <typename T>
+void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s);
 // Solve: forward then backward sweep in place on B (one persistent launch each).
 // flags: ceil(nrhs/16) * nblk words that persist across calls; tickets: 2 ints.
-void launch_solve_persistent(const double *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
+template <typename T>
+void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, uint32_t *flags, int *tickets, uint32_t epoch_fwd,
                              uint32_t epoch_bwd, const int *info, int *status, hipStream_t s);
 // Symmetric-storage helpers.
@@ -57,5 +47,20 @@ void launch_upper_to_lower(const double *A, int64_t lda, int64_t n, double *W, i
 void launch_lower_to_upper(const double *W, int64_t ldw, int64_t n, double *A, int64_t lda, hipStream_t s);
 void launch_sy_l1norm(const double *A, int64_t lda, int64_t n, bool upper, double *colsum, double *out,
                       hipStream_t s);
+
+// Mixed-precision solve helpers (mixed.hip).
+int64_t sym_tiles64(int64_t n);  // lower 64 x 64 tiles: size of the norm partials
+int residual_chunks(int64_t n, int64_t nrhs);  // k-chunks of the residual (partials: chunks*n*nrhs)
+// W (lower) = lower triangle of symmetric A (lower or upper storage), anorm = ||A||_F.
+template <typename T>
+void launch_sym_copy_lower(const double *A, int64_t lda, int64_t n, bool upper, T *W, int64_t ldw, double *part,
+                           double *anorm, hipStream_t s);
+// R = B - A X (A symmetric, FP64, one triangle), rr/xx = column sums of squares of R and X,
+// berr = max_j ||R_j|| / (anorm ||X_j||).
+void launch_residual(const double *A, int64_t lda, int64_t n, bool upper, const double *X, int64_t ldx,
+                     const double *B, int64_t ldb, int64_t nrhs, double *R, int64_t ldr, double *part, double *rr,
+                     double *xx, const double *anorm, double *berr, hipStream_t s);
+void launch_add_in_place(double *X, int64_t ldx, const double *D, int64_t ldd, int64_t n, int64_t nrhs,
+                         hipStream_t s);
 
 }  // namespace mi

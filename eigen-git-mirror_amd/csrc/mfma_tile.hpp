@@ -21,6 +21,7 @@ namespace mi {
 template <int TA, int TB, int D = 3, bool MASKED = false>
 struct WaveTile {
     static constexpr int U = D + 1;  // rolling buffer depth
+    static constexpr int KS = 4;     // k per MFMA step
     dvec4 acc[TA][TB];
     const double *pa[TA];
     const double *qb[TB];
@@ -164,6 +165,162 @@ struct WaveTile {
     // i = row0 + 16a + (lane % 16), j = col0 + 16b + 4r + lane / 16.
     __device__ __forceinline__ static int row_of(int a) { return 16 * a + (lane_id() & 15); }
     __device__ __forceinline__ static int col_of(int b, int r) { return 16 * b + 4 * r + (lane_id() >> 4); }
+};
+
+// The same engine for FP32 storage with low-precision MFMA (FP32 accumulation), used by the
+// mixed-precision factorization: PREC = kPrecF32 (v_mfma_f32_16x16x4_f32, one k per lane
+// per step), kPrecBF16 (v_mfma_f32_16x16x32_bf16, 8 consecutive k per lane, operands
+// rounded to BF16 when issued) or kPrecBF16x3 (each operand split x = hi + lo into two BF16
+// parts; hi*hi + lo*hi + hi*lo on three MFMAs, ~16 significant bits per product).  The
+// depth is a multiple of KS (no masking); result element r of tile (a, b) is
+// C[row0 + 16a + lane%16][col0 + 16b + 4*(lane/16) + r].
+template <int TA, int TB, int PREC>
+struct WaveTileF {
+    static constexpr int E = PREC == kPrecF32 ? 1 : 8;  // k per lane per MFMA
+    static constexpr int KS = 4 * E;                     // k per MFMA step
+    static constexpr int D = E == 1 ? 3 : 1;             // steps of loads in flight
+    static constexpr int U = D + 1;
+    fvec4 acc[TA][TB];
+    const float *pa[TA];
+    const float *qb[TB];
+    int64_t ldp, ldq;
+
+    __device__ __forceinline__ void zero()
+    {
+#pragma unroll
+        for (int a = 0; a < TA; ++a)
+#pragma unroll
+            for (int b = 0; b < TB; ++b) acc[a][b] = fvec4{0.f, 0.f, 0.f, 0.f};
+    }
+
+    __device__ __forceinline__ void setup(const float *P, int64_t ldp_, int p0, int pmax, const float *Q,
+                                          int64_t ldq_, int q0, int qmax, int /*kmax*/)
+    {
+        const int l = lane_id();
+        ldp = ldp_;
+        ldq = ldq_;
+#pragma unroll
+        for (int a = 0; a < TA; ++a) {
+            int r = p0 + 16 * a + (l & 15);
+            r = r < pmax ? r : pmax - 1;
+            pa[a] = P + r;
+        }
+#pragma unroll
+        for (int b = 0; b < TB; ++b) {
+            int r = q0 + 16 * b + (l & 15);
+            r = r < qmax ? r : qmax - 1;
+            qb[b] = Q + r;
+        }
+    }
+
+    struct Frag {
+        float p[TA][E], q[TB][E];
+    };
+
+    __device__ __forceinline__ void load(Frag &f, int s, int S) const
+    {
+        s = s < S ? s : S - 1;
+        const int kb = KS * s + E * (lane_id() >> 4);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int64_t op = (int64_t)(kb + e) * ldp, oq = (int64_t)(kb + e) * ldq;
+#pragma unroll
+            for (int a = 0; a < TA; ++a) f.p[a][e] = pa[a][op];
+#pragma unroll
+            for (int b = 0; b < TB; ++b) f.q[b][e] = qb[b][oq];
+        }
+    }
+
+    __device__ __forceinline__ void mma(const Frag &f, int)
+    {
+        if constexpr (PREC == kPrecF32) {
+#pragma unroll
+            for (int a = 0; a < TA; ++a)
+#pragma unroll
+                for (int b = 0; b < TB; ++b) acc[a][b] = mfma_f32(f.q[b][0], f.p[a][0], acc[a][b]);
+        } else {
+            bf16x8 ph[TA], qh[TB], pl[TA], ql[TB];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+#pragma unroll
+                for (int a = 0; a < TA; ++a) {
+                    ph[a][e] = (__bf16)f.p[a][e];
+                    if (PREC == kPrecBF16x3) pl[a][e] = (__bf16)(f.p[a][e] - (float)ph[a][e]);
+                }
+#pragma unroll
+                for (int b = 0; b < TB; ++b) {
+                    qh[b][e] = (__bf16)f.q[b][e];
+                    if (PREC == kPrecBF16x3) ql[b][e] = (__bf16)(f.q[b][e] - (float)qh[b][e]);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < TA; ++a)
+#pragma unroll
+                for (int b = 0; b < TB; ++b) {
+                    if (PREC == kPrecBF16x3) {
+                        acc[a][b] = mfma_bf16(ql[b], ph[a], acc[a][b]);
+                        acc[a][b] = mfma_bf16(qh[b], pl[a], acc[a][b]);
+                    }
+                    acc[a][b] = mfma_bf16(qh[b], ph[a], acc[a][b]);
+                }
+        }
+    }
+
+    template <int S>
+    __device__ __forceinline__ void run_fixed()
+    {
+        Frag buf[U];
+#pragma unroll
+        for (int u = 0; u < D; ++u) load(buf[u], u, S);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (s + D < S) load(buf[(s + D) % U], s + D, S);
+            mma(buf[s % U], s);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    template <bool ASSIGN, bool LOWER>
+    __device__ __forceinline__ void store(float *C, int64_t ldc, int row0, int col0, int imax, int jmax) const
+    {
+        float old[TA][TB][4];
+        if (!ASSIGN) {
+#pragma unroll
+            for (int a = 0; a < TA; ++a)
+#pragma unroll
+                for (int b = 0; b < TB; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        int i = row0 + row_of(a), j = col0 + col_of(b, r);
+                        i = i < imax ? i : imax - 1;
+                        j = j < jmax ? j : jmax - 1;
+                        old[a][b][r] = C[i + (int64_t)j * ldc];
+                    }
+        }
+#pragma unroll
+        for (int a = 0; a < TA; ++a)
+#pragma unroll
+            for (int b = 0; b < TB; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = row0 + row_of(a), j = col0 + col_of(b, r);
+                    const bool ok = i < imax && j < jmax && (!LOWER || j <= i);
+                    if (ok) C[i + (int64_t)j * ldc] = ASSIGN ? acc[a][b][r] : old[a][b][r] - acc[a][b][r];
+                }
+    }
+
+    __device__ __forceinline__ static int row_of(int a) { return 16 * a + (lane_id() & 15); }
+    __device__ __forceinline__ static int col_of(int b, int r) { return 16 * b + 4 * (lane_id() >> 4) + r; }
+};
+
+// Tile engine for storage type T and trailing-update arithmetic PREC.
+template <typename T, int PREC>
+struct TileSel {
+    using type = WaveTile<4, 4>;
+};
+template <int PREC>
+struct TileSel<float, PREC> {
+    using type = WaveTileF<4, 4, PREC>;
 };
 
 // Workgroup -> tile remap that gives each XCD a contiguous range of the tile sequence

@@ -94,13 +94,14 @@ constexpr int kSmem = 128 * 129 + 8 * 256;
 static_assert(kSmem >= 128 * LDL, "LDS buffer too small for the block");
 
 // A (global, ld lda, valid order bs) -> S (lower part, identity beyond bs); 32 loads in
-// flight per thread, from clamped in-bounds addresses.
-__device__ __forceinline__ void block_to_lds(double *S, const double *__restrict__ Ab, int64_t lda, int bs)
+// flight per thread, from clamped in-bounds addresses.  T = float widens to FP64.
+template <typename T>
+__device__ __forceinline__ void block_to_lds(double *S, const T *__restrict__ Ab, int64_t lda, int bs)
 {
     const int tid = threadIdx.x, r = tid & 127, ch = tid >> 7;
 #pragma unroll
     for (int q0 = 0; q0 < 64; q0 += 32) {
-        double a[32];
+        T a[32];
 #pragma unroll
         for (int q = 0; q < 32; ++q) {
             const int c = 2 * (q0 + q) + ch;
@@ -111,18 +112,19 @@ __device__ __forceinline__ void block_to_lds(double *S, const double *__restrict
         for (int q = 0; q < 32; ++q) {
             const int c = 2 * (q0 + q) + ch;
             const bool in = r < bs && c < bs;
-            if (r >= c) S[r + c * LDL] = in ? a[q] : (r == c ? 1.0 : 0.0);
+            if (r >= c) S[r + c * LDL] = in ? (double)a[q] : (r == c ? 1.0 : 0.0);
         }
     }
 }
 
-__device__ __forceinline__ void lds_to_block(const double *S, double *__restrict__ Ab, int64_t lda, int bs)
+template <typename T>
+__device__ __forceinline__ void lds_to_block(const double *S, T *__restrict__ Ab, int64_t lda, int bs)
 {
     const int tid = threadIdx.x, r = tid & 127, ch = tid >> 7;
 #pragma unroll 8
     for (int q = 0; q < 64; ++q) {
         const int c = 2 * q + ch;
-        if (r >= c && r < bs) Ab[r + (int64_t)c * lda] = S[r + c * LDL];
+        if (r >= c && r < bs) Ab[r + (int64_t)c * lda] = (T)S[r + c * LDL];
     }
 }
 
@@ -190,10 +192,11 @@ __device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl, 
 // Rows 0..127 of P (valid m) <- P * L^{-T} with L (128 x 128 lower, ld ldl, valid order bs,
 // identity beyond) and its 8 leaf inverses Xl, using smem for L and the leaves.  Each wave
 // owns 2 row tiles of 16 rows.  IDENTITY: P is the identity (not read); the result L^{-T}
-// goes to O (ld 128) and its transpose L^{-1} to O + 128*128.
-template <bool IDENTITY>
-__device__ __forceinline__ void trsm_rows(double *smem, double *__restrict__ P, int64_t ldp, int m,
-                                          const double *__restrict__ L, int64_t ldl, int bs,
+// goes to O (ld 128) and its transpose L^{-1} to O + 128*128.  T = float: P and L are read
+// as FP32 and widened, the arithmetic is FP64, the result is rounded back to FP32.
+template <bool IDENTITY, typename T>
+__device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64_t ldp, int m,
+                                          const T *__restrict__ L, int64_t ldl, int bs,
                                           const double *__restrict__ Xl, double *__restrict__ O)
 {
     const int l = lane_id(), w = threadIdx.x >> 6;
@@ -203,7 +206,7 @@ __device__ __forceinline__ void trsm_rows(double *smem, double *__restrict__ P, 
     for (int e = threadIdx.x; e < 128 * 128; e += 256) {
         const int r = e & 127, c = e >> 7;
         const bool in = r < bs && c < bs;
-        const double v = L[(in ? r : 0) + (int64_t)(in ? c : 0) * ldl];
+        const double v = (double)L[(in ? r : 0) + (int64_t)(in ? c : 0) * ldl];
         Ls[r + 129 * c] = (r >= c) ? (in ? v : (r == c ? 1.0 : 0.0)) : 0.0;
     }
     for (int e = threadIdx.x; e < 8 * 256; e += 256) Xs[e] = Xl[e];
@@ -222,7 +225,7 @@ __device__ __forceinline__ void trsm_rows(double *smem, double *__restrict__ P, 
                 if (IDENTITY) {
                     acc[t][r] = (i == j) ? 1.0 : 0.0;
                 } else {
-                    const double pv = P[(i < m ? i : m - 1) + (int64_t)j * ldp];
+                    const double pv = (double)P[(i < m ? i : m - 1) + (int64_t)j * ldp];
                     acc[t][r] = (i < m) ? pv : 0.0;
                 }
             }
@@ -249,8 +252,6 @@ __device__ __forceinline__ void trsm_rows(double *smem, double *__restrict__ P, 
         sol[0][p] = z[0];
         sol[1][p] = z[1];
     }
-    double *Out = IDENTITY ? O : P;
-    const int64_t ldo = IDENTITY ? 128 : ldp;
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int i = row0 + 16 * t + (l & 15);
@@ -260,16 +261,29 @@ __device__ __forceinline__ void trsm_rows(double *smem, double *__restrict__ P, 
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int j = 16 * p + 4 * r + (l >> 4);
-                    Out[i + (int64_t)j * ldo] = sol[t][p][r];
-                    if (IDENTITY) Out[128 * 128 + j + 128 * i] = sol[t][p][r];
+                    if (IDENTITY) {
+                        O[i + (int64_t)j * 128] = sol[t][p][r];
+                        O[128 * 128 + j + 128 * i] = sol[t][p][r];
+                    } else {
+                        P[i + (int64_t)j * ldp] = (T)sol[t][p][r];
+                    }
                 }
         }
     }
 }
 
-// The panel role of one block step (see panel_fused_kernel); g is the role from the ticket.
-// smem: kSmem doubles of LDS; s_fail/s_ok: LDS ints.
-__device__ __forceinline__ void panel_role(int g, double *smem, int *s_fail, int *s_ok, double *__restrict__ A,
+// The panel role of one block step; g is the role from the ticket:
+//   role 0   : C = A_kk - P_0 P_0^T straight into LDS, Cholesky in LDS, L11 and the leaf
+//              inverses to memory, then publish `flag` = epoch (release);
+//   role g>0 : rows k0+128g..+127: C = A_gk - P_g P_0^T written back, wait for the flag,
+//              then rows <- rows * L11^{-T}.
+// The diagonal role publishes even on a failing pivot, so that no waiter is left behind;
+// waiters then see `info` and stop.  (LLT.h:350-366: the update of block column k by all
+// earlier panels precedes its unblocked factor and the right-side solve.)
+// smem: kSmem doubles of LDS; s_fail/s_ok: LDS ints.  T/PREC: storage and update arithmetic
+// (the diagonal factor and the panel solve are FP64 for every T).
+template <typename T, int PREC>
+__device__ __forceinline__ void panel_role(int g, double *smem, int *s_fail, int *s_ok, T *__restrict__ A,
                                            int64_t lda, int64_t n, int64_t k0, double *__restrict__ Wleaf, int *info,
                                            uint32_t *flag, uint32_t epoch, int *status)
 {
@@ -277,23 +291,24 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_fail, int
     const int64_t R0 = k0 + 128 * (int64_t)g;
     const int rows = (int)((n - R0) < 128 ? (n - R0) : 128);
     const int bs = (int)((n - k0) < 128 ? (n - k0) : 128);
-    double *Ck = A + R0 + k0 * lda;  // this role's 128 rows of block column k0
+    T *Ck = A + R0 + k0 * lda;  // this role's 128 rows of block column k0
     double *Xl = Wleaf + (size_t)(k0 / 128) * kLeafDoubles;
 
     if (k0 > 0) {
         // C -= P_g P_0^T, P = previous panel (columns k0-128..k0-1): rows R0.. and k0..
         const int wr = w >> 1, wc = w & 1;
         const bool active = !(g == 0 && wr == 0 && wc == 1);  // strictly upper quadrant of the diagonal tile
-        WaveTile<4, 4> t;
+        using Tile = typename TileSel<T, PREC>::type;
+        Tile t;
         t.zero();
         if (active) {
             t.setup(A + R0 + (k0 - 128) * lda, lda, wr * 64, rows, A + k0 + (k0 - 128) * lda, lda, wc * 64, bs, 128);
-            t.run_fixed<32>();
+            t.template run_fixed<128 / Tile::KS>();
         }
         if (g == 0) {
             double *S = smem;
             if (active) {  // S = A_kk - acc on the lower part, identity beyond bs
-                double old[4][4][4];
+                T old[4][4][4];
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -314,12 +329,12 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_fail, int
                             const int i = wr * 64 + t.row_of(a), j = wc * 64 + t.col_of(b, r);
                             if (i >= j) {
                                 const bool in = i < bs && j < bs;
-                                S[i + j * LDL] = in ? old[a][b][r] - t.acc[a][b][r] : (i == j ? 1.0 : 0.0);
+                                S[i + j * LDL] = in ? (double)old[a][b][r] - (double)t.acc[a][b][r] : (i == j ? 1.0 : 0.0);
                             }
                         }
             }
         } else {
-            t.store<false, false>(Ck, lda, wr * 64, wc * 64, rows, 128);
+            t.template store<false, false>(Ck, lda, wr * 64, wc * 64, rows, 128);
         }
     } else if (g == 0) {
         block_to_lds(smem, Ck, lda, bs);
@@ -353,7 +368,7 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_fail, int
     }
     __syncthreads();
     if (!*s_ok || factor_failed(info)) return;
-    trsm_rows<false>(smem, Ck, lda, rows, A + k0 + k0 * lda, lda, 128, Xl, nullptr);
+    trsm_rows<false, T>(smem, Ck, lda, rows, A + k0 + k0 * lda, lda, 128, Xl, nullptr);
 }
 
 }  // namespace mi

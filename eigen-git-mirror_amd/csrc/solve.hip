@@ -53,8 +53,9 @@ __device__ __forceinline__ bool wait_flag(const uint32_t *flag, uint32_t epoch, 
 
 // BACKWARD = false: forward sweep with L; true: backward sweep with L^T.
 // Winv holds, per block b, X_b^T (ld 128) at Winv + b*2*16384 and X_b at +16384.
-template <bool BACKWARD>
-__global__ __launch_bounds__(512) void solve_sweep_kernel(const double *__restrict__ L, int64_t lda, int64_t n,
+// T: storage of L (float for a mixed-precision factor, widened on load; FP64 arithmetic).
+template <bool BACKWARD, typename T>
+__global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ L, int64_t lda, int64_t n,
                                                           double *__restrict__ B, int64_t ldb, int64_t nrhs,
                                                           const double *__restrict__ Winv, uint32_t *flags,
                                                           int *ticket, uint32_t epoch, const int *info, int *status)
@@ -99,15 +100,15 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const double *__restri
             const int64_t j0_ = (int64_t)(jb_)*SB;                                                 \
             const int bj_ = (int)((n - j0_) < SB ? (n - j0_) : SB);                                \
             if (bj_ == SB) {                                                                       \
-                const double *p_ = BACKWARD ? L + (j0_ + kl) + (i0 + rc) * lda                     \
-                                            : L + (i0 + rc) + (j0_ + kl) * lda;                    \
-                _Pragma("unroll") for (int s = 0; s < SB / 4; ++s) { pv[s] = *p_; p_ += pstep; }   \
+                const T *p_ = BACKWARD ? L + (j0_ + kl) + (i0 + rc) * lda                          \
+                                       : L + (i0 + rc) + (j0_ + kl) * lda;                         \
+                _Pragma("unroll") for (int s = 0; s < SB / 4; ++s) { pv[s] = (double)*p_; p_ += pstep; } \
             } else {                                                                               \
                 _Pragma("unroll") for (int s = 0; s < SB / 4; ++s)                                 \
                 {                                                                                  \
                     int k = 4 * s + kl;                                                            \
                     k = k < bj_ ? k : bj_ - 1;                                                     \
-                    pv[s] = BACKWARD ? L[(j0_ + k) + (i0 + rc) * lda] : L[(i0 + rc) + (j0_ + k) * lda]; \
+                    pv[s] = (double)(BACKWARD ? L[(j0_ + k) + (i0 + rc) * lda] : L[(i0 + rc) + (j0_ + k) * lda]); \
                 }                                                                                  \
             }                                                                                      \
         } else {                                                                                   \
@@ -168,7 +169,8 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const double *__restri
 __global__ void reset_ticket_kernel(int *ticket) { *ticket = 0; }
 
 // flags: ngrp * nblk uint32 (persistent across calls, epochs increase); tickets: 2 ints.
-void launch_solve_persistent(const double *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
+template <typename T>
+void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, uint32_t *flags, int *tickets, uint32_t epoch_fwd,
                              uint32_t epoch_bwd, const int *info, int *status, hipStream_t s)
 {
@@ -176,11 +178,18 @@ void launch_solve_persistent(const double *L, int64_t lda, int64_t n, double *B,
     const int ngrp = (int)((nrhs + SC - 1) / SC);
     const unsigned items = (unsigned)(nblk * ngrp);
     hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, s, tickets);
-    hipLaunchKernelGGL(solve_sweep_kernel<false>, dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv, flags,
+    hipLaunchKernelGGL((solve_sweep_kernel<false, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv, flags,
                        tickets, epoch_fwd, info, status);
     hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, s, tickets + 1);
-    hipLaunchKernelGGL(solve_sweep_kernel<true>, dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv, flags,
+    hipLaunchKernelGGL((solve_sweep_kernel<true, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv, flags,
                        tickets + 1, epoch_bwd, info, status);
 }
+
+template void launch_solve_persistent<double>(const double *, int64_t, int64_t, double *, int64_t, int64_t,
+                                              const double *, uint32_t *, int *, uint32_t, uint32_t, const int *,
+                                              int *, hipStream_t);
+template void launch_solve_persistent<float>(const float *, int64_t, int64_t, double *, int64_t, int64_t,
+                                             const double *, uint32_t *, int *, uint32_t, uint32_t, const int *,
+                                             int *, hipStream_t);
 
 }  // namespace mi

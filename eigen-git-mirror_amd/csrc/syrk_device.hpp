@@ -13,17 +13,19 @@
 namespace mi {
 
 // Tile (ti, tj) of C (m x m, lower), columns restricted to j < ncol; depth K (compile time).
-template <int K>
-__device__ __forceinline__ void syrk_tile(double *__restrict__ C, int64_t ldc, const double *__restrict__ P,
-                                          int64_t ldp, int m, int ncol, int ti, int tj)
+// T = double: FP64 MFMA; T = float: FP32 storage with the low-precision MFMA of PREC.
+template <int K, typename T = double, int PREC = kPrecF64>
+__device__ __forceinline__ void syrk_tile(T *__restrict__ C, int64_t ldc, const T *__restrict__ P, int64_t ldp,
+                                          int m, int ncol, int ti, int tj)
 {
+    using Tile = typename TileSel<T, PREC>::type;
     const int w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
     if (ti == tj && wr == 0 && wc == 1) return;
     const int i0 = ti * 128 + wr * 64, j0 = tj * 128 + wc * 64;
-    WaveTile<4, 4> t;
+    Tile t;
     t.zero();
     t.setup(P, ldp, i0, m, P, ldp, j0, m, K);
-    t.template run_fixed<K / 4>();
+    t.template run_fixed<K / Tile::KS>();
     t.template store<false, true>(C, ldc, i0, j0, m, ncol);
 }
 

@@ -57,6 +57,9 @@ def lib() -> ctypes.CDLL:
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]
         L.mi_llt_trace_reset.argtypes = [_VP]
         L.mi_llt_status.argtypes = [_VP, ctypes.POINTER(_INT)]
+        L.mi_dposv_mixed.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _INT,
+                                     ctypes.c_double, _INT, ctypes.POINTER(_INT), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(_INT)]
         L.LAPACKE_dpotrf.argtypes = [_INT, ctypes.c_char, _INT, _VP, _INT]
         L.LAPACKE_dpotrs.argtypes = [_INT, ctypes.c_char, _INT, _INT, _VP, _INT, _VP, _INT]
         L.dpotrf_.argtypes = [ctypes.c_char_p, ctypes.POINTER(_INT), _VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]
@@ -64,7 +67,8 @@ def lib() -> ctypes.CDLL:
                               ctypes.POINTER(_INT), _VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]
         for name in ("mi_llt_create", "mi_llt_destroy", "mi_llt_set_stream", "mi_dpotrf", "mi_dpotrf_async",
                      "mi_dpotrs", "mi_dposv_async", "mi_dsy_l1norm", "LAPACKE_dpotrf", "LAPACKE_dpotrs",
-                     "mi_llt_trace", "mi_llt_trace_read", "mi_llt_trace_reset", "mi_llt_status"):
+                     "mi_llt_trace", "mi_llt_trace_read", "mi_llt_trace_reset", "mi_llt_status",
+                     "mi_dposv_mixed"):
             getattr(L, name).restype = _INT
         L.dpotrf_.restype = None
         L.dpotrs_.restype = None
@@ -199,6 +203,31 @@ def posv_async_(a: torch.Tensor, b: torch.Tensor, d_info: torch.Tensor, uplo: st
                                  _ld(b), _VP(d_info.data_ptr())), "mi_dposv_async")
 
 
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16x3": 3}
+
+
+def posv_mixed(a: torch.Tensor, b: torch.Tensor, x: torch.Tensor | None = None, prec: str = "fp32",
+               tol: float = 1e-14, maxit: int = 10, uplo: str = "L", handle: Handle | None = None):
+    """Mixed-precision solve with FP64 iterative refinement (``mi_dposv_mixed``).
+
+    ``a`` (float64, column-major, ``uplo`` triangle) is not modified; returns
+    ``(x, iters, berr, info)``: the FP64 solution (written into ``x`` when given), the number
+    of refinement corrections (-1 / -2: the FP64 factorization was used because the
+    low-precision factor was not positive definite / refinement did not reach ``tol``), the
+    final normwise backward error and LAPACK ``info``.  Synchronizes.
+    """
+    h = handle.bind_stream() if handle else default_handle()
+    n = int(a.shape[0])
+    nrhs = int(b.shape[1])
+    if x is None:
+        x = torch.empty_strided((n, nrhs), (1, max(n, 1)), dtype=torch.float64, device=a.device)
+    it, be, inf = ctypes.c_int(0), ctypes.c_double(0.0), ctypes.c_int(0)
+    h.check(lib().mi_dposv_mixed(h.ptr, _uplo(uplo), n, nrhs, _VP(a.data_ptr()), _ld(a), _VP(b.data_ptr()), _ld(b),
+                                 _VP(x.data_ptr()), _ld(x), PREC[prec], float(tol), int(maxit), ctypes.byref(it),
+                                 ctypes.byref(be), ctypes.byref(inf)), "mi_dposv_mixed")
+    return x, int(it.value), float(be.value), int(inf.value)
+
+
 def sy_l1norm(a: torch.Tensor, uplo: str = "L", handle: Handle | None = None) -> torch.Tensor:
     """``m_l1_norm`` of LLT::compute (LLT.h:442-453) as a 0-d float64 device tensor."""
     h = handle.bind_stream() if handle else default_handle()
@@ -211,4 +240,4 @@ def sy_l1norm(a: torch.Tensor, uplo: str = "L", handle: Handle | None = None) ->
 from .llt import LLT, ComputationInfo  # noqa: E402
 
 __all__ = ["lib", "colmajor", "Handle", "default_handle", "potrf_", "potrf_async_", "potrs_", "posv_async_",
-           "sy_l1norm", "LLT", "ComputationInfo", "MiLLTError", "LIB_PATH"]
+           "posv_mixed", "PREC", "sy_l1norm", "LLT", "ComputationInfo", "MiLLTError", "LIB_PATH"]

@@ -70,6 +70,29 @@ def spectrum_spd(n: int, kappa: float, seed: int = 0, n_reflectors: int = 64) ->
     return np.asfortranarray(a)
 
 
+def spectrum_spd_torch(n: int, kappa: float, seed: int = 0, n_reflectors: int = 64, device="cuda"):
+    """Same matrix as :func:`spectrum_spd`, formed as H_1 (... (H_m diag(lambda) H_m) ...) H_1 with
+    two-sided reflector updates (O(m n^2), no dense product), column-major float64 on ``device``:
+    for H = I - tau v v^T, p = tau A v, w = p - (tau/2)(v^T p) v, A <- A - v w^T - w v^T."""
+    import torch
+
+    lam = torch.from_numpy(spectrum(n, kappa, seed)).to(device)
+    rng = np.random.default_rng(seed + 2000)
+    vs = torch.from_numpy(rng.standard_normal((n_reflectors, n))).to(device)
+    a = torch.diag(lam)
+    for i in range(n_reflectors - 1, -1, -1):
+        v = vs[i]
+        tau = 2.0 / torch.dot(v, v)
+        p = tau * (a @ v)
+        w = p - (0.5 * tau * torch.dot(v, p)) * v
+        a -= torch.outer(v, w)
+        a -= torch.outer(w, v)
+    a = 0.5 * (a + a.t())
+    out = torch.empty_strided((n, n), (1, n), dtype=torch.float64, device=device)
+    out.copy_(a)
+    return out
+
+
 def indefinite_at(n: int, k: int, seed: int = 0) -> np.ndarray:
     """A copy of gram_spd(n) whose pivot k is made negative: a_kk is replaced by
     ||L[k, :k]||^2 - 1, so the leading minors of order <= k are PD and order k+1 is not

@@ -108,6 +108,23 @@ int mi_dposv_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, double *
 int mi_dsy_l1norm(mi_llt_handle h, char uplo, int64_t n, const double *dA, int64_t lda,
                   double *d_norm);
 
+/* ---- mixed precision + FP64 iterative refinement --------------------------------------
+ * No reference counterpart: LLT computes in one precision.  Solves A X = B for symmetric
+ * positive definite A (FP64, `uplo` triangle at dA, not modified) and writes X to dX
+ * (n x nrhs, ld ldx).  The factorization runs on an FP32 working copy of A: trailing
+ * updates in the MFMA arithmetic `prec` (mi_prec) with FP32 accumulation, the diagonal-block
+ * factor and the panel solves in FP64.  Refinement in FP64: X = solve(B); then repeatedly
+ * R = B - A X, stop when max_j ||R_j||_2 / (||A||_F ||X_j||_2) <= tol, else X += solve(R),
+ * at most `maxit` corrections.  When the low-precision factor is not positive definite, or
+ * refinement does not reach tol, the plain FP64 factorization solves the system instead
+ * (*iters = -1, resp. -2; cf. LAPACK dsposv's negative ITER).  prec = MI_PREC_FP64 factors
+ * in FP64 at once (*iters = 0).  *iters: corrections applied; *berr: the final backward
+ * error; *info: 0, or k+1 when the FP64 factor fails at column k (A not positive definite,
+ * X then unchanged).  Host pointers iters/berr/info; the call waits for the stream. */
+int mi_dposv_mixed(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double *dA, int64_t lda,
+                   const double *dB, int64_t ldb, double *dX, int64_t ldx, int prec, double tol, int maxit,
+                   int *iters, double *berr, int *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,7 @@ def declared_functions():
 
 def test_headers_declare_the_boundary():
     names = declared_functions()
-    for must in ("mi_llt_create", "mi_dpotrf", "mi_dpotrs", "mi_dposv_async", "LAPACKE_dpotrf",
+    for must in ("mi_llt_create", "mi_dpotrf", "mi_dpotrs", "mi_dposv_async", "mi_dposv_mixed", "LAPACKE_dpotrf",
                  "LAPACKE_dpotrs", "dpotrf_", "dpotrs_"):
         assert must in names
 
@@ -45,6 +45,9 @@ def test_null_handle_is_rejected():
     assert lib.mi_dpotrs(None, b"L", 4, 1, None, 4, None, 4) == -1003
     assert lib.mi_llt_set_stream(None, None) == -1003
     assert lib.mi_llt_last_error(None) == b"invalid handle"
+    it, be = ctypes.c_int(0), ctypes.c_double(0.0)
+    assert lib.mi_dposv_mixed(None, b"L", 4, 1, None, 4, None, 4, None, 4, 1, 1e-14, 10, ctypes.byref(it),
+                              ctypes.byref(be), ctypes.byref(info)) == -1003
 
 
 def test_lapacke_argument_checks():
