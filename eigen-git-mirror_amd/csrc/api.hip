@@ -42,6 +42,12 @@ struct mi_llt_context {
     };
     std::vector<Rec> recs;
     double t_ms[MI_TRACE_KINDS] = {0};
+    // batched path: its own grow-only workspace (stable across calls of the same or smaller
+    // size, so a captured graph keeps valid pointers) and the schedule parameters
+    void *vb_ws = nullptr;
+    size_t vb_bytes = 0;
+    int vb_wg_per_cu = 2, vb_group = 4, vb_lpt = 1;
+    int cus = 0;
     double t_flops[MI_TRACE_KINDS] = {0};
     int64_t t_calls[MI_TRACE_KINDS] = {0};
 };
@@ -212,7 +218,7 @@ void block_inverses(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, const d
 // Forward + backward substitution in place on B with a factor whose block inverses are in winv.
 template <typename T>
 int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, int64_t ldb, int64_t nrhs,
-               const double *winv, const int *d_info)
+               const double *winv, const int *d_info, int sweeps = 3)
 {
     const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
     const int64_t ngrp = (nrhs + 15) / 16;
@@ -224,9 +230,10 @@ int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, i
         h->epoch = 0;
     }
     const uint32_t ef = ++h->epoch, eb = ++h->epoch;
-    traced(h, h->stream, MI_TRACE_SOLVE, 2.0 * (double)n * (double)n * (double)nrhs, [&] {
+    const double per_sweep = (double)n * (double)n * (double)nrhs;
+    traced(h, h->stream, MI_TRACE_SOLVE, per_sweep * ((sweeps & 1) + ((sweeps >> 1) & 1)), [&] {
         mi::launch_solve_persistent<T>(L, ldl, n, B, ldb, nrhs, winv, h->sync + kSyncHeader, (int *)h->sync + 1, ef,
-                                       eb, d_info, (int *)h->sync, h->stream);
+                                       eb, d_info, (int *)h->sync, h->stream, sweeps);
     });
     return 0;
 }
@@ -288,6 +295,56 @@ int prepare_mixed(mi_llt_handle h, int64_t n, int64_t nrhs, bool fp64, MixedLayo
     return 0;
 }
 
+template <typename T>
+int vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n, T *const *d_A, const int *d_lda,
+             const int *d_nrhs, T *const *d_B, const int *d_ldb, int *d_info)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (batch < 0) return -3;
+    if (batch == 0) return 0;
+    if (!d_n) return -4;
+    if (!d_A) return -5;
+    if (!d_lda) return -6;
+    if (!d_nrhs) return -7;
+    if (!d_B) return -8;
+    if (!d_ldb) return -9;
+    if (!d_info) return -10;
+    int64_t grid = (int64_t)h->cus * h->vb_wg_per_cu;
+    const int64_t need_wg = (batch + 3) / 4;  // one system per worker wave at least
+    if (grid > need_wg) grid = need_wg;
+    if (grid < 1) grid = 1;
+    const size_t order_b = round_up((size_t)batch * sizeof(int));
+    const size_t ctr_b = 256;
+    const size_t scr_b = round_up((size_t)mi::vb_scratch_elems((int)grid) * sizeof(T));
+    const size_t need = order_b + ctr_b + scr_b;
+    if (need > h->vb_bytes) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
+        if (h->vb_ws) hipFree(h->vb_ws);
+        h->vb_ws = nullptr;
+        h->vb_bytes = 0;
+        if (hipMalloc(&h->vb_ws, need) != hipSuccess) {
+            h->vb_ws = nullptr;
+            snprintf(h->err, sizeof h->err, "batched workspace of %zu bytes", need);
+            return MI_LLT_E_ALLOC;
+        }
+        h->vb_bytes = need;
+    }
+    char *p = (char *)h->vb_ws;
+    int *order = (int *)p;
+    int *counter = (int *)(p + order_b);
+    T *scratch = (T *)(p + order_b + ctr_b);
+    const double fl_est = 0.0;
+    traced(h, h->stream, MI_TRACE_BATCH, fl_est, [&] {
+        mi::launch_vbatched<T>(batch, d_n, d_A, d_lda, d_nrhs, d_B, d_ldb, d_info, order, counter, scratch, (int)grid,
+                               h->vb_group, h->vb_lpt, h->stream);
+    });
+    return check_launch(h, "vbatched");
+}
+
 int read_back(mi_llt_handle h, void *dst, const void *src, size_t bytes)
 {
     hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream);
@@ -305,6 +362,8 @@ int mi_llt_create(mi_llt_handle *h)
     mi_llt_context *c = new (std::nothrow) mi_llt_context();
     if (!c) return MI_LLT_E_ALLOC;
     hipGetDevice(&c->device);
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || c->cus <= 0)
+        c->cus = 256;
     if (hipMalloc(&c->d_info, 256) != hipSuccess) {
         delete c;
         return MI_LLT_E_ALLOC;
@@ -318,6 +377,7 @@ int mi_llt_destroy(mi_llt_handle h)
     if (!valid(h)) return MI_LLT_E_HANDLE;
     hipStreamSynchronize(h->stream);
     if (h->sync) hipFree(h->sync);
+    if (h->vb_ws) hipFree(h->vb_ws);
     if (h->ws) hipFree(h->ws);
     if (h->d_info) hipFree(h->d_info);
     h->magic = 0;
@@ -606,6 +666,109 @@ int mi_dposv_mixed(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const do
         mi::launch_add_in_place(dX, ldx, m.D, n, n, nrhs, s);
     }
     return fp64_path(-2);  // refinement did not reach tol
+}
+
+int mi_dposv_vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n, double *const *d_A,
+                      const int *d_lda, const int *d_nrhs, double *const *d_B, const int *d_ldb, int *d_info)
+{
+    return vbatched<double>(h, uplo, batch, d_n, d_A, d_lda, d_nrhs, d_B, d_ldb, d_info);
+}
+
+int mi_sposv_vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n, float *const *d_A,
+                      const int *d_lda, const int *d_nrhs, float *const *d_B, const int *d_ldb, int *d_info)
+{
+    return vbatched<float>(h, uplo, batch, d_n, d_A, d_lda, d_nrhs, d_B, d_ldb, d_info);
+}
+
+int mi_llt_set_vbatched_config(mi_llt_handle h, int wg_per_cu, int group, int longest_first)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    if (wg_per_cu < 1 || wg_per_cu > 64) return -2;
+    if (group != 2 && group != 4) return -3;
+    h->vb_wg_per_cu = wg_per_cu;
+    h->vb_group = group;
+    h->vb_lpt = longest_first ? 1 : 0;
+    return 0;
+}
+
+int mi_llt_get_vbatched_config(mi_llt_handle h, int *wg_per_cu, int *group, int *longest_first)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    if (wg_per_cu) *wg_per_cu = h->vb_wg_per_cu;
+    if (group) *group = h->vb_group;
+    if (longest_first) *longest_first = h->vb_lpt;
+    return 0;
+}
+
+int mi_dpotrf_adjoint(mi_llt_handle h, char uplo, int64_t n, const double *dL, int64_t ldl, double *dG, int64_t ldg)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (n < 0) return -3;
+    if (dL == nullptr && n > 0) return -4;
+    if (ldl < (n > 1 ? n : 1)) return -5;
+    if (dG == nullptr && n > 0) return -6;
+    if (ldg < (n > 1 ? n : 1)) return -7;
+    if (n == 0) return 0;
+    hipStream_t s = h->stream;
+    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
+    const size_t leaf = round_up((size_t)nblk * mi::kLeafDoubles * sizeof(double));
+    const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
+    const size_t sq = round_up((size_t)n * (size_t)n * sizeof(double));
+    int rc = reserve(h, leaf + inv + 2 * sq);
+    if (rc) return rc;
+    double *wleaf = (double *)h->ws;
+    double *winv = (double *)((char *)h->ws + leaf);
+    double *W2 = (double *)((char *)h->ws + leaf + inv);
+    double *W3 = (double *)((char *)h->ws + leaf + inv + sq);
+    const double nd = (double)n;
+    traced(h, s, MI_TRACE_GEMM, nd * nd * nd / 3.0, [&] { mi::launch_trmm_tn_lower(dL, ldl, dG, ldg, n, W3, n, s); });
+    mi::launch_sym_tile(0, W3, n, n, W2, n, s);           // M
+    mi::launch_leaf_inverses<double>(dL, ldl, n, wleaf, s);  // recomputed from L
+    block_inverses<double>(h, dL, ldl, n, wleaf, winv, nullptr);
+    if ((rc = solve_only<double>(h, dL, ldl, n, W2, n, n, winv, nullptr, 2))) return rc;  // Z = L^{-T} M
+    mi::launch_sym_tile(1, W2, n, n, W3, n, s);           // Z^T
+    if ((rc = solve_only<double>(h, dL, ldl, n, W3, n, n, winv, nullptr, 2))) return rc;  // S = L^{-T} Z^T
+    mi::launch_sym_tile(2, W3, n, n, dG, ldg, s);         // Abar = (S + S^T) / 2
+    return check_launch(h, "potrf_adjoint");
+}
+
+int mi_dpotrs_adjoint(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double *dL, int64_t ldl,
+                      const double *dX, int64_t ldx, double *dXbar, int64_t ldxb, double *dLbar, int64_t ldlb)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (dL == nullptr && n > 0) return -5;
+    if (ldl < (n > 1 ? n : 1)) return -6;
+    if (dX == nullptr && n > 0 && nrhs > 0) return -7;
+    if (ldx < (n > 1 ? n : 1)) return -8;
+    if (dXbar == nullptr && n > 0 && nrhs > 0) return -9;
+    if (ldxb < (n > 1 ? n : 1)) return -10;
+    if (dLbar == nullptr && n > 0) return -11;
+    if (ldlb < (n > 1 ? n : 1)) return -12;
+    if (n == 0 || nrhs == 0) return 0;
+    hipStream_t s = h->stream;
+    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
+    const size_t leaf = round_up((size_t)nblk * mi::kLeafDoubles * sizeof(double));
+    const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
+    const size_t vec = round_up((size_t)n * (size_t)nrhs * sizeof(double));
+    int rc = reserve(h, leaf + inv + 2 * vec);
+    if (rc) return rc;
+    double *wleaf = (double *)h->ws;
+    double *winv = (double *)((char *)h->ws + leaf);
+    double *U = (double *)((char *)h->ws + leaf + inv);
+    double *V = (double *)((char *)h->ws + leaf + inv + vec);
+    mi::launch_leaf_inverses<double>(dL, ldl, n, wleaf, s);
+    block_inverses<double>(h, dL, ldl, n, wleaf, winv, nullptr);
+    if ((rc = solve_only<double>(h, dL, ldl, n, dXbar, ldxb, nrhs, winv, nullptr))) return rc;  // Bbar
+    mi::launch_solve_adjoint_update(dL, ldl, n, nrhs, dX, ldx, dXbar, ldxb, dLbar, ldlb, U, V, s);
+    return check_launch(h, "potrs_adjoint");
 }
 
 }  // extern "C"

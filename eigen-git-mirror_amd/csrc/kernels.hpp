@@ -38,10 +38,11 @@ template <typename T>
 void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s);
 // Solve: forward then backward sweep in place on B (one persistent launch each).
 // flags: ceil(nrhs/16) * nblk words that persist across calls; tickets: 2 ints.
+// sweeps: 1 forward only (L Y = B), 2 backward only (L^T X = B), 3 both.
 template <typename T>
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, uint32_t *flags, int *tickets, uint32_t epoch_fwd,
-                             uint32_t epoch_bwd, const int *info, int *status, hipStream_t s);
+                             uint32_t epoch_bwd, const int *info, int *status, hipStream_t s, int sweeps = 3);
 // Symmetric-storage helpers.
 void launch_upper_to_lower(const double *A, int64_t lda, int64_t n, double *W, int64_t ldw, hipStream_t s);
 void launch_lower_to_upper(const double *W, int64_t ldw, int64_t n, double *A, int64_t lda, hipStream_t s);
@@ -62,5 +63,25 @@ void launch_residual(const double *A, int64_t lda, int64_t n, bool upper, const 
                      double *xx, const double *anorm, double *berr, hipStream_t s);
 void launch_add_in_place(double *X, int64_t ldx, const double *D, int64_t ldd, int64_t n, int64_t nrhs,
                          hipStream_t s);
+
+// Batched uneven factor + solve (vbatched.hip): plan (argument checks, longest-first order,
+// counter reset) + one persistent launch of `grid` workgroups of 4 independent waves.
+// order: batch ints; counter: 1 int; scratch: vb_scratch_elems(grid) elements.
+int64_t vb_scratch_elems(int grid);
+template <typename T>
+void launch_vbatched(int64_t batch, const int *d_n, T *const *d_A, const int *d_lda, const int *d_nrhs,
+                     T *const *d_B, const int *d_ldb, int *d_info, int *order, int *counter, T *scratch, int grid,
+                     int group, int lpt, hipStream_t s);
+
+// Reverse mode (adjoint.hip).
+// P (lower) = tril(L^T G) for lower L, G.
+void launch_trmm_tn_lower(const double *L, int64_t ldl, const double *G, int64_t ldg, int64_t n, double *P,
+                          int64_t ldp, hipStream_t s);
+// mode 0: out = symmetric completion of tril(in), halved; 1: out = in^T; 2: out = (in + in^T)/2.
+void launch_sym_tile(int mode, const double *in, int64_t ldi, int64_t n, double *out, int64_t ldo, hipStream_t s);
+// Lbar (lower) -= tril(Bb (L^T X)^T + X (L^T Bb)^T); U, V: n x r scratch.
+void launch_solve_adjoint_update(const double *L, int64_t ldl, int64_t n, int64_t r, const double *X, int64_t ldx,
+                                 const double *Bb, int64_t ldbb, double *Lb, int64_t ldlb, double *U, double *V,
+                                 hipStream_t s);
 
 }  // namespace mi

@@ -172,24 +172,28 @@ __global__ void reset_ticket_kernel(int *ticket) { *ticket = 0; }
 template <typename T>
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, uint32_t *flags, int *tickets, uint32_t epoch_fwd,
-                             uint32_t epoch_bwd, const int *info, int *status, hipStream_t s)
+                             uint32_t epoch_bwd, const int *info, int *status, hipStream_t s, int sweeps)
 {
     const int nblk = (int)((n + SB - 1) / SB);
     const int ngrp = (int)((nrhs + SC - 1) / SC);
     const unsigned items = (unsigned)(nblk * ngrp);
-    hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, s, tickets);
-    hipLaunchKernelGGL((solve_sweep_kernel<false, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv, flags,
-                       tickets, epoch_fwd, info, status);
-    hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, s, tickets + 1);
-    hipLaunchKernelGGL((solve_sweep_kernel<true, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv, flags,
-                       tickets + 1, epoch_bwd, info, status);
+    if (sweeps & 1) {
+        hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, s, tickets);
+        hipLaunchKernelGGL((solve_sweep_kernel<false, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv,
+                           flags, tickets, epoch_fwd, info, status);
+    }
+    if (sweeps & 2) {
+        hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, s, tickets + 1);
+        hipLaunchKernelGGL((solve_sweep_kernel<true, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv,
+                           flags, tickets + 1, epoch_bwd, info, status);
+    }
 }
 
 template void launch_solve_persistent<double>(const double *, int64_t, int64_t, double *, int64_t, int64_t,
                                               const double *, uint32_t *, int *, uint32_t, uint32_t, const int *,
-                                              int *, hipStream_t);
+                                              int *, hipStream_t, int);
 template void launch_solve_persistent<float>(const float *, int64_t, int64_t, double *, int64_t, int64_t,
                                              const double *, uint32_t *, int *, uint32_t, uint32_t, const int *,
-                                             int *, hipStream_t);
+                                             int *, hipStream_t, int);
 
 }  // namespace mi

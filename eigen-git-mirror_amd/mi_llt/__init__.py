@@ -57,6 +57,13 @@ def lib() -> ctypes.CDLL:
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]
         L.mi_llt_trace_reset.argtypes = [_VP]
         L.mi_llt_status.argtypes = [_VP, ctypes.POINTER(_INT)]
+        L.mi_dposv_vbatched.argtypes = [_VP, ctypes.c_char, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
+        L.mi_sposv_vbatched.argtypes = [_VP, ctypes.c_char, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
+        L.mi_llt_set_vbatched_config.argtypes = [_VP, _INT, _INT, _INT]
+        L.mi_llt_get_vbatched_config.argtypes = [_VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT),
+                                                 ctypes.POINTER(_INT)]
+        L.mi_dpotrf_adjoint.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, _VP, _I64]
+        L.mi_dpotrs_adjoint.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _VP, _I64]
         L.mi_dposv_mixed.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _INT,
                                      ctypes.c_double, _INT, ctypes.POINTER(_INT), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(_INT)]
@@ -68,7 +75,8 @@ def lib() -> ctypes.CDLL:
         for name in ("mi_llt_create", "mi_llt_destroy", "mi_llt_set_stream", "mi_dpotrf", "mi_dpotrf_async",
                      "mi_dpotrs", "mi_dposv_async", "mi_dsy_l1norm", "LAPACKE_dpotrf", "LAPACKE_dpotrs",
                      "mi_llt_trace", "mi_llt_trace_read", "mi_llt_trace_reset", "mi_llt_status",
-                     "mi_dposv_mixed"):
+                     "mi_dposv_mixed", "mi_dposv_vbatched", "mi_sposv_vbatched", "mi_llt_set_vbatched_config",
+                     "mi_llt_get_vbatched_config", "mi_dpotrf_adjoint", "mi_dpotrs_adjoint"):
             getattr(L, name).restype = _INT
         L.dpotrf_.restype = None
         L.dpotrs_.restype = None
@@ -129,7 +137,7 @@ class Handle:
         self.check(lib().mi_llt_status(self._h, ctypes.byref(v)), "mi_llt_status")
         return int(v.value)
 
-    TRACE_KINDS = {"factor_step": 0, "trsm": 1, "syrk": 2, "solve": 3, "trtri": 4, "panel_update": 5}
+    TRACE_KINDS = {"factor_step": 0, "solve": 3, "trtri": 4, "residual": 5, "batched": 6}
 
     def trace(self, enable: bool = True):
         self.check(lib().mi_llt_trace(self._h, int(enable)), "mi_llt_trace")

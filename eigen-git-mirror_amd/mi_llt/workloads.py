@@ -105,3 +105,20 @@ def indefinite_at(n: int, k: int, seed: int = 0) -> np.ndarray:
     y = np.linalg.solve(l, a[:k, k])
     a[k, k] = y @ y - 1.0
     return np.asfortranarray(a)
+
+
+def gp_kernel_torch(n: int, d: int = 8, ell: float = 1.5, sigma2: float = 1e-2, seed: int = 0, device="cuda"):
+    """The GP workload: X n x d iid N(0,1) (PCG64 ``seed``), K = exp(-||x_i - x_j||^2 / (2 ell^2))
+    + sigma2 I, y ~ N(0,1) (seed + 1).  Returns (K column-major float64, y)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    x = torch.from_numpy(rng.standard_normal((n, d))).to(device)
+    y = torch.from_numpy(np.random.default_rng(seed + 1).standard_normal(n)).to(device)
+    sq = (x * x).sum(1)
+    d2 = (sq[:, None] + sq[None, :] - 2.0 * (x @ x.t())).clamp_min(0.0)
+    k = torch.exp(-d2 / (2.0 * ell * ell))
+    k.diagonal().add_(sigma2)
+    out = torch.empty_strided((n, n), (1, n), dtype=torch.float64, device=device)
+    out.copy_(0.5 * (k + k.t()))
+    return out, y

@@ -73,7 +73,8 @@ int mi_llt_status(mi_llt_handle h, int *status);
 #define MI_TRACE_SYRK 2    /* trailing A22 -= A21 A21^T */
 #define MI_TRACE_SOLVE 3   /* forward + backward substitution */
 #define MI_TRACE_TRTRI 4   /* diagonal-block inverses L_bb^{-T} built for the solve */
-#define MI_TRACE_GEMM 5    /* lookahead: update of the next panel by the current one */
+#define MI_TRACE_GEMM 5    /* FP64 residual R = B - A X of the mixed-precision refinement */
+#define MI_TRACE_BATCH 6   /* batched factor + solve (plan + persistent launch) */
 #define MI_TRACE_KINDS 8
 int mi_llt_trace(mi_llt_handle h, int enable);
 int mi_llt_trace_read(mi_llt_handle h, int kind, double *ms, double *flops, int64_t *launches);
@@ -124,6 +125,42 @@ int mi_dsy_l1norm(mi_llt_handle h, char uplo, int64_t n, const double *dA, int64
 int mi_dposv_mixed(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double *dA, int64_t lda,
                    const double *dB, int64_t ldb, double *dX, int64_t ldx, int prec, double tol, int maxit,
                    int *iters, double *berr, int *info);
+
+/* ---- batched, uneven sizes ------------------------------------------------------------
+ * Counterpart: a loop of LLT<MatrixXd>(A_i).solve(B_i) (LLT.h:104-110, 430-460, 483-498) over
+ * independent systems.  Factors each A_i (n_i x n_i, lower triangle, in place) and overwrites
+ * B_i (n_i x nrhs_i) with A_i^{-1} B_i, for i < batch, in one persistent launch (plus a
+ * one-workgroup plan launch); nothing is allocated when the handle's batched workspace is
+ * already large enough, so repeated calls can be captured in a HIP graph.  All arrays are
+ * device arrays of `batch` entries.  0 <= n_i <= 256; uplo 'L' only (MI_LLT_E_UNSUPPORTED
+ * for 'U').  d_info[i]: 0, k+1 for a failing pivot at column k (B_i unchanged), or -1 n_i out
+ * of range, -2 lda_i < max(1, n_i), -3 nrhs_i < 0, -4 ldb_i < max(1, n_i) (system skipped). */
+int mi_dposv_vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n, double *const *d_A,
+                      const int *d_lda, const int *d_nrhs, double *const *d_B, const int *d_ldb, int *d_info);
+/* Same in FP32 (FP32 storage and arithmetic, FP32 MFMA). */
+int mi_sposv_vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n, float *const *d_A,
+                      const int *d_lda, const int *d_nrhs, float *const *d_B, const int *d_ldb, int *d_info);
+/* Schedule of the batched launch (no reference counterpart; set by the autotuner in
+ * mi_llt.batched): workgroups of 4 worker waves per CU (1..64), tile rows per register block
+ * of the panel update (2 or 4), and longest-first ordering (1) or the given order (0). */
+int mi_llt_set_vbatched_config(mi_llt_handle h, int wg_per_cu, int group, int longest_first);
+int mi_llt_get_vbatched_config(mi_llt_handle h, int *wg_per_cu, int *group, int *longest_first);
+
+/* ---- reverse mode -----------------------------------------------------------------------
+ * No reference counterpart (Eigen does not differentiate LLT).  For A = L L^T, uplo 'L'
+ * (MI_LLT_E_UNSUPPORTED for 'U'):
+ * mi_dpotrf_adjoint: dG holds Lbar = dloss/dL on entry (its lower triangle is read) and Abar =
+ *   dloss/dA on return, as a full symmetric matrix (both triangles written; the symmetric
+ *   convention of torch.linalg.cholesky): Abar = sym(L^{-T} M L^{-1}), M the halved symmetric
+ *   completion of tril(L^T Lbar).  The diagonal-block inverses of L are recomputed, not
+ *   stored by the forward pass.  Device workspace: 2 n^2 doubles.  Asynchronous.
+ * mi_dpotrs_adjoint: for X = A^{-1} B computed with the factor L: dXbar holds Xbar on entry
+ *   and Bbar = A^{-1} Xbar on return; dLbar (lower triangle, ld ldlb) is accumulated:
+ *   Lbar -= tril(Bbar (L^T X)^T + X (L^T Bbar)^T).  Asynchronous. */
+int mi_dpotrf_adjoint(mi_llt_handle h, char uplo, int64_t n, const double *dL, int64_t ldl, double *dG,
+                      int64_t ldg);
+int mi_dpotrs_adjoint(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double *dL, int64_t ldl,
+                      const double *dX, int64_t ldx, double *dXbar, int64_t ldxb, double *dLbar, int64_t ldlb);
 
 #ifdef __cplusplus
 }
