@@ -331,16 +331,34 @@ __global__ __launch_bounds__(64 * kVbWaves) void vb_persistent_kernel(int64_t ba
     const int w = threadIdx.x >> 6, l = lane_id();
     const int worker = blockIdx.x * kVbWaves + w;
     T *Xg = scratch + (size_t)worker * kVbMaxT * 256;
-    for (;;) {
-        int t = 0;
-        if (l == 0) t = atomicAdd(counter, 1);
-        t = __builtin_amdgcn_readfirstlane(t);
+    // The loop has no lane-dependent branch: every value that steers it is wave-uniform
+    // (scalar).  The ticket is drawn by all lanes adding (lane == 0) -- one atomic add of 1
+    // after the wave reduction -- and read from lane 0; the per-system parameters go through
+    // readfirstlane; the info store is done by every lane (same address, same value).  (A
+    // branch on `lane == 0` both at the top and at the bottom of the body lets the compiler
+    // merge the two across the back edge and split the loop so that lanes leave it one at a
+    // time, with the others re-reading a ticket they never drew: that never terminates.)
+    // Each pass draws one ticket and tickets only grow, so no worker needs more than batch + 1
+    // passes; the bound keeps the loop finite whatever the counter holds.
+    for (int64_t pass = 0; pass <= batch; ++pass) {
+        int t = atomicAdd(counter, l == 0 ? 1 : 0);
+        t = __builtin_amdgcn_readlane(t, 0);
         if (t >= batch) break;
-        const int i = order[t];
-        if (d_info[i] != 0) continue;  // invalid arguments (set by the plan)
-        const int n = d_n[i];
-        const int info = vb_system<T, G>(n, d_A[i], d_lda[i], d_nrhs[i], d_B[i], d_ldb[i], lds[w][0], lds[w][1], Xg);
-        if (l == 0) d_info[i] = info;
+        const int i = __builtin_amdgcn_readfirstlane(order[t]);
+        const int pre = __builtin_amdgcn_readfirstlane(d_info[i]);  // argument error set by the plan
+        if (pre == 0) {
+            const int n = __builtin_amdgcn_readfirstlane(d_n[i]);
+            const int lda = __builtin_amdgcn_readfirstlane(d_lda[i]);
+            const int nr = __builtin_amdgcn_readfirstlane(d_nrhs[i]);
+            const int ldb = __builtin_amdgcn_readfirstlane(d_ldb[i]);
+            const uint64_t pa = (uint64_t)(uintptr_t)d_A[i], pb = (uint64_t)(uintptr_t)d_B[i];
+            T *A = (T *)(uintptr_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(pa >> 32)) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pa));
+            T *B = (T *)(uintptr_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(pb >> 32)) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pb));
+            const int info = vb_system<T, G>(n, A, lda, nr, B, ldb, lds[w][0], lds[w][1], Xg);
+            d_info[i] = info;
+        }
     }
 }
 
