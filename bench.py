@@ -12,8 +12,13 @@ TFLOP/s = [F(n) + 2 n^2 r] / time, F(n) = n(n+1)(2n+1)/6 (LAPACK / Eigen bench c
 bench/benchCholesky.cpp:34-39).
 
 Also reported: normwise backward errors of the last step, the dominant kernel's roofline
-(the trailing SYRK, from live HIP events around each launch in separate traced steps), and
+(factor_step_kernel, from live HIP events around each launch in separate traced steps), and
 the genuine Eigen LLT timed on this host's cores (``cpu_baseline``).
+
+``legs`` (skipped with --no-legs) measures the other workloads of the path, each on its own
+configuration: the mixed-precision solve with FP64 refinement against the plain FP64 path
+at n=16384 (kappa 1e2, 1e4, 1e6, 16 right-hand sides), the batched uneven solve of 20,000
+systems (graph replay), and the Gaussian-process adjoint (n=4096, and 256 systems of 512).
 """
 from __future__ import annotations
 
@@ -70,6 +75,174 @@ def cpu_baseline(a_host, b_host, n, r):
             "seconds": t}
 
 
+def _time_ms(fn, reps, setup=None):
+    """Median of ``reps`` event-timed calls (``setup`` runs before each, untimed)."""
+    import torch
+
+    ts = []
+    for _ in range(reps):
+        if setup:
+            setup()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ts.sort()
+    return ts[len(ts) // 2], ts[0]
+
+
+def leg_mixed(n=16384, r=16, kappas=(1e2, 1e4, 1e6), reps=3):
+    """Mixed-precision LLT + FP64 refinement vs the plain FP64 factor+solve, same inputs.
+    Throughput counts FP64-equivalent flops F(n) + 2 n^2 r for every precision; the mixed
+    time includes the FP32 copy, refinement steps and their FP64 residuals."""
+    import torch
+
+    import mi_llt
+    from mi_llt import workloads
+
+    dev = torch.device("cuda:0")
+    fl = flops_factor(n) + flops_solve(n, r)
+    b = mi_llt.colmajor(torch.from_numpy(workloads.rhs(n, r, seed=1)).to(dev))
+    out = {"n": n, "nrhs": r, "tol": 1e-14, "cases": []}
+    h = mi_llt.Handle().bind_stream()
+    for kappa in kappas:
+        a = workloads.spectrum_spd_torch(n, kappa, seed=0, device=dev)
+        # plain FP64: copy into working storage + factor + solve (mi_dposv_async)
+        da = torch.empty_strided((n, n), (1, n), dtype=torch.float64, device=dev)
+        xb = torch.empty_strided((n, r), (1, n), dtype=torch.float64, device=dev)
+        d_info = torch.zeros(1, dtype=torch.int32, device=dev)
+
+        def plain():
+            da.copy_(a)
+            xb.copy_(b)
+            mi_llt.posv_async_(da, xb, d_info, handle=h)
+
+        plain()
+        t64, _ = _time_ms(plain, reps)
+        asym = torch.tril(a) + torch.tril(a, -1).t()
+        be64 = float(((asym @ xb - b).norm(dim=0) / (torch.linalg.matrix_norm(asym) * xb.norm(dim=0))).max())
+        del da, asym
+        case = {"kappa": kappa, "fp64": {"ms": round(t64, 3), "tflops": round(fl / t64 / 1e9, 3), "berr": be64}}
+        best = None
+        for prec in ("fp32", "bf16x3", "bf16"):
+            res = {}
+
+            def mixed():
+                res["r"] = mi_llt.posv_mixed(a, b, prec=prec, tol=1e-14, maxit=30, handle=h)
+
+            mixed()
+            tm, _ = _time_ms(mixed, reps)
+            x, iters, berr, info = res["r"]
+            case[prec] = {"ms": round(tm, 3), "tflops": round(fl / tm / 1e9, 3), "iters": iters, "berr": berr,
+                          "speedup_vs_fp64": round(t64 / tm, 3), "info": info}
+            if info == 0 and berr <= 1e-14 and (best is None or tm < case[best]["ms"]):
+                best = prec
+        case["best"] = best
+        out["cases"].append(case)
+        del a
+        torch.cuda.empty_cache()
+    out["speedup_best"] = {str(c["kappa"]): (c[c["best"]]["speedup_vs_fp64"] if c["best"] else None)
+                           for c in out["cases"]}
+    return out
+
+
+def leg_batched(count=20000, reps=5):
+    """20,000 uneven SPD systems (n ~ U{8..256}, 1..8 right-hand sides) factored and solved in
+    one persistent launch, replayed from a HIP graph; inputs restored between replays
+    (untimed).  Roofline: per system max(bytes / 8 TB/s, flops / peak), summed."""
+    import numpy as np
+    import torch
+
+    import mi_llt
+    from mi_llt import batched
+
+    dev = torch.device("cuda:0")
+    out = {"count": count}
+    for dt, peak in ((torch.float64, PEAK_FP64_TFLOPS), (torch.float32, 157.3)):
+        vb = batched.gram_batch(count, seed=0, dtype=dt, device=dev)
+        a0, b0 = vb.a_flat.clone(), vb.b_flat.clone()
+        h = mi_llt.Handle().bind_stream()
+
+        def restore():
+            vb.a_flat.copy_(a0)
+            vb.b_flat.copy_(b0)
+
+        tune = batched.autotune(vb, h, candidates=[(2, 4, True), (3, 2, True), (2, 2, True), (4, 2, True)],
+                                reps=2, cache=None, )
+        restore()
+        batched.posv_vbatched_(vb, h)
+        torch.cuda.synchronize()
+        restore()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            batched.posv_vbatched_(vb, h)
+        tm, tb = _time_ms(g.replay, reps, setup=restore)
+        bad = int((vb.d_info != 0).sum())
+        n, r = vb.ns.astype(np.float64), vb.rs.astype(np.float64)
+        es = 8 if dt == torch.float64 else 4
+        sys_bytes = ((n * (n + 1) / 2) * 2 + 2 * n * r) * es
+        sys_flops = n * (n + 1) * (2 * n + 1) / 6 + 2 * n * n * r
+        t_roof = float(np.maximum(sys_bytes / (PEAK_HBM_GBS * 1e9), sys_flops / (peak * 1e12)).sum()) * 1e3
+        out["fp64" if dt == torch.float64 else "fp32"] = {
+            "ms": round(tm, 4), "best_ms": round(tb, 4), "systems_per_s": round(count / (tm * 1e-3), 1),
+            "tflops": round(vb.flops() / tm / 1e9, 3), "algorithmic_GBps": round(vb.bytes() / tm / 1e6, 1),
+            "roofline_ms": round(t_roof, 4), "frac_of_roofline": round(t_roof / tm, 4), "failed": bad,
+            "config": {"wg_per_cu": tune["config"][0], "group": tune["config"][1], "longest_first": tune["config"][2]},
+            "graph": True}
+        del vb, a0, b0, g
+        torch.cuda.empty_cache()
+    return out
+
+
+def leg_adjoint(n=4096, nb=512, batch=256, reps=3):
+    """Gaussian-process log marginal likelihood and its gradient dL/dK through the library's
+    factorization, solve and hand-written adjoints (torch.autograd.Function)."""
+    import torch
+
+    from mi_llt import autograd as mad
+    from mi_llt import workloads
+
+    dev = torch.device("cuda:0")
+    out = {}
+    k, y = workloads.gp_kernel_torch(n, seed=0, device=dev)
+
+    def fwd_bwd():
+        k1 = k.detach().requires_grad_(True)
+        mad.gp_log_marginal(k1, y).backward()
+        return k1.grad
+
+    fwd_bwd()
+    t_all, _ = _time_ms(fwd_bwd, reps)
+
+    def fwd():
+        with torch.no_grad():
+            mad.gp_log_marginal(k, y)
+
+    t_f, _ = _time_ms(fwd, reps)
+    # adjoint flops: tril(L^T Lbar) n^3/3 + two triangular solves with n right-hand sides 2 n^3
+    adj_fl = n ** 3 / 3 + 2.0 * n ** 3
+    t_adj = max(t_all - t_f, 1e-6)
+    out["single"] = {"n": n, "forward_ms": round(t_f, 3), "forward_backward_ms": round(t_all, 3),
+                     "adjoint_ms": round(t_adj, 3), "adjoint_tflops": round(adj_fl / t_adj / 1e9, 3),
+                     "adjoint_frac_fp64_peak": round(adj_fl / t_adj / 1e9 / PEAK_FP64_TFLOPS, 4)}
+    ks = [workloads.gp_kernel_torch(nb, seed=100 + i, device=dev) for i in range(batch)]
+
+    def batch_all():
+        for kk, yy in ks:
+            k1 = kk.detach().requires_grad_(True)
+            mad.gp_log_marginal(k1, yy).backward()
+
+    batch_all()
+    tb, _ = _time_ms(batch_all, 1)
+    out["batch"] = {"count": batch, "n": nb, "forward_backward_ms": round(tb, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +251,7 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-legs", action="store_true", help="skip the mixed / batched / adjoint legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X and the path is built for one GPU "
@@ -165,6 +339,12 @@ def main():
         log("[bench] cpu baseline (Eigen LLT on the host)")
         cpu = cpu_baseline(np.asfortranarray(a.cpu().numpy()), np.asfortranarray(b.cpu().numpy()), n, r)
 
+    legs = None
+    if not args.no_legs:
+        del da, db, a, b
+        torch.cuda.empty_cache()
+        legs = {"mixed": leg_mixed(), "batched": leg_batched(), "adjoint": leg_adjoint()}
+
     out = {
         "metric": METRIC, "value": round(tflops, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -178,6 +358,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "wall_s_timed": round(t_wall, 4),
+        "legs": legs,
     }
     print(json.dumps(out))
 

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void factor_step_kernel(S *__restrict__ A, int
                                                           uint32_t *flag, uint32_t epoch, int *status)
 {
     __shared__ double smem[kSmem];
-    __shared__ int s_role, s_fail, s_ok;
+    __shared__ int s_role, s_ok;
     // every workgroup draws its ticket, even when it then stops: the host counts them
     if (threadIdx.x == 0) s_role = (int)(atomicAdd(tickets, 1u) - ticket_base);
     __syncthreads();
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void factor_step_kernel(S *__restrict__ A, int
     const int t = s_role;
     const int nsyrk = T * (T + 1) / 2;
     if (Gp > 0 && (t == 0 || t > nsyrk)) {  // ticket 0: diagonal role; last Gp-1: row roles
-        panel_role<S, PREC>(t == 0 ? 0 : t - nsyrk, smem, &s_fail, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch,
+        panel_role<S, PREC>(t == 0 ? 0 : t - nsyrk, smem, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch,
                             status);
         return;
     }

@@ -18,74 +18,59 @@ __device__ __forceinline__ double readlane_d(double x, int lane)
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Wave-level factor of the panel S[c0.., c0..c0+15] (rows c0..127): lane l holds rows
-// c0+l and c0+l+64 in v0/v1.  Returns the failing column (panel-local) or -1; on success
-// the panel is written back to S and v0 of lanes 0..15 keeps the leaf L_pp (row l).
-// Scaling by 1/sqrt(x) is a multiplication by the reciprocal of the stored diagonal.
-__device__ __forceinline__ int panel_factor(double *S, int c0, double (&v0)[16])
+// Panel p (columns c0..c0+15 of the 128 x 128 block in S, ld LDL), all four waves at once.
+// Every wave holds the 16 x 16 leaf in lanes 0..15 (row c0+l) and runs the column loop of
+// LLT.h:320-334 on it (pivot test, sqrt, scale, rank-1 update) redundantly, so its lane
+// broadcasts (readlane) need no barrier.  Lanes 16..63 of waves 0..2 hold the 48 rows
+// c0+16+48w+(l-16) below the leaf and receive the same column operations, which is the panel
+// solve A_ip L_pp^{-T}; lanes 16..31 of wave 3 hold the rows of the 16 x 16 identity, which the
+// same operations turn into L_pp^{-T}, i.e. column i of X_pp = L_pp^{-1} in lane 16+i, written
+// to xleaf column-major.  1/sqrt from the hardware estimate and one Newton step, d = x/sqrt(x).
+// Returns the failing panel column or -1, the same in every wave.
+__device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restrict__ xleaf)
 {
-    const int l = lane_id();
-    const int r0 = c0 + l, r1 = c0 + l + 64;
-    double v1[16];
+    const int l = lane_id(), w = threadIdx.x >> 6;
+    const bool leaf = l < 16;
+    const int rb = c0 + 16 + 48 * w + (l - 16);
+    const bool below = !leaf && w < 3 && rb < 128;
+    const bool ident = !leaf && w == 3 && l < 32;
+    const int r = leaf ? c0 + l : (below ? rb : c0);
+    double v[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
-        const double a0 = S[(r0 < 128 ? r0 : 127) + (c0 + c) * LDL];
-        const double a1 = S[(r1 < 128 ? r1 : 127) + (c0 + c) * LDL];
-        v0[c] = (r0 < 128 && (l >= 16 || c <= l)) ? a0 : 0.0;
-        v1[c] = (r1 < 128) ? a1 : 0.0;
+        const double a = S[r + (c0 + c) * LDL];
+        v[c] = leaf ? (c <= l ? a : 0.0) : (below ? a : ((ident && c == l - 16) ? 1.0 : 0.0));
     }
+    int fail = -1;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const double x = readlane_d(v0[j], j);  // pivot: row c0+j lives in lane j
-        if (x <= 0.0) return j;
-        // 1/sqrt(x) from the hardware estimate and one Newton step (relative error ~1 ulp),
-        // d = x * rd; the chain is what serialises the 128 columns of the block
+        const double x = readlane_d(v[j], j);  // pivot: leaf row j lives in lane j
+        if (fail < 0 && x <= 0.0) fail = j;    // LLT.h:327-330; a NaN pivot passes
         double rd = __builtin_amdgcn_rsq(x);
         rd = rd * (1.5 - 0.5 * x * rd * rd);
         const double d = x * rd;
-        if (l == j) v0[j] = d;
-        if (l > j) v0[j] *= rd;
-        v1[j] *= rd;
+        v[j] = l == j ? d : (l > j ? v[j] * rd : v[j]);
 #pragma unroll
         for (int c = j + 1; c < 16; ++c) {
-            const double lcj = readlane_d(v0[j], c);  // L(c0+c, c0+j)
-            v0[c] -= v0[j] * lcj;
-            v1[c] -= v1[j] * lcj;
+            const double lcj = readlane_d(v[j], c);  // L(c0+c, c0+j)
+            v[c] -= v[j] * lcj;
         }
     }
+    if (fail >= 0) return fail;
+    if (leaf && w == 0) {
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        if (r0 < 128 && (l >= 16 || c <= l)) S[r0 + (c0 + c) * LDL] = v0[c];
-        if (r1 < 128) S[r1 + (c0 + c) * LDL] = v1[c];
+        for (int c = 0; c < 16; ++c)
+            if (c <= l) S[r + (c0 + c) * LDL] = v[c];
+    }
+    if (below) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) S[r + (c0 + c) * LDL] = v[c];
+    }
+    if (ident) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) xleaf[c + 16 * (l - 16)] = v[c];
     }
     return -1;
-}
-
-// X = L_pp^{-1} of the 16 x 16 leaf held by lanes 0..15 (row l of L_pp in v0), written to
-// xleaf (column-major, zeros above the diagonal).  Lane i builds row i of X.
-__device__ __forceinline__ void leaf_inverse(const double (&v0)[16], double *__restrict__ xleaf)
-{
-    const int l = lane_id();
-    double x[16];
-#pragma unroll
-    for (int c = 0; c < 16; ++c) x[c] = (c == l) ? 1.0 : 0.0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const double rk = 1.0 / readlane_d(v0[k], k);
-        if (l == k) {
-#pragma unroll
-            for (int c = 0; c <= k; ++c) x[c] *= rk;
-        }
-#pragma unroll
-        for (int c = 0; c <= k; ++c) {
-            const double xkc = readlane_d(x[c], k);
-            if (l > k) x[c] -= v0[k] * xkc;
-        }
-    }
-    if (l < 16) {
-#pragma unroll
-        for (int c = 0; c < 16; ++c) xleaf[l + 16 * c] = (c <= l) ? x[c] : 0.0;
-    }
 }
 
 // LDS of one workgroup, shared by the two roles: the 128 x 128 block (ld LDL) for the
@@ -128,59 +113,50 @@ __device__ __forceinline__ void lds_to_block(const double *S, T *__restrict__ Ab
     }
 }
 
-// Cholesky of the 128 x 128 lower block in S (all 256 threads).  Returns the first failing
-// column or -1 (uniform across the workgroup); writes the 8 leaf inverses to Xl.
-__device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl, int *s_fail)
+// Cholesky of the 128 x 128 lower block in S (all 256 threads), in eight 16-column panels:
+// panel factor and solve by all four waves (panel_factor4), then the rank-16 update of the
+// rest of the block by the panel on FP64 MFMA from LDS, the 28..1 lower 16 x 16 tiles dealt
+// round-robin to the waves, all MFMAs before the read-modify-writes.  Returns the first
+// failing column or -1 (uniform); writes the 8 leaf inverses to Xl.
+__device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
 {
-    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
-    if (tid == 0) *s_fail = -1;
+    const int w = threadIdx.x >> 6, l = lane_id();
     __syncthreads();
     for (int p = 0; p < 8; ++p) {
         const int c0 = 16 * p;
-        double v0[16];
-        if (w == 0) {
-            const int f = panel_factor(S, c0, v0);
-            if (f >= 0 && l == 0) *s_fail = c0 + f;
-        }
-        __syncthreads();
-        if (*s_fail >= 0) return *s_fail;
-        if (w == 0) {
-            leaf_inverse(v0, Xl + p * 256);
-        } else {
-            // rank-16 update of S[c0+16.., c0+16..] (lower) by the panel columns c0..c0+15:
-            // waves 1..3 take tiles w-1, w+2, ... (at most 10); all MFMAs first (the panel
-            // columns are not written here), then the read-modify-writes
-            const int T = 7 - p;
-            const int nt = T * (T + 1) / 2;
-            dvec4 acc[10];
-            int tr[10], tc[10];
+        const int f = panel_factor4(S, c0, Xl + p * 256);
+        if (f >= 0) return c0 + f;
+        __syncthreads();  // the panel's columns are in S
+        const int T = 7 - p;
+        const int nt = T * (T + 1) / 2;
+        dvec4 acc[7];
+        int tr[7], tc[7];
 #pragma unroll
-            for (int h = 0; h < 10; ++h) {
-                const int t = (w - 1) + 3 * h;
-                int ti = 0, tj = 0;
-                if (t < nt) lower_tile(t, ti, tj);
-                tr[h] = c0 + 16 + 16 * ti;
-                tc[h] = c0 + 16 + 16 * tj;
-                acc[h] = dvec4{0.0, 0.0, 0.0, 0.0};
-                if (t < nt) {
+        for (int h = 0; h < 7; ++h) {
+            const int t = w + 4 * h;
+            int ti = 0, tj = 0;
+            if (t < nt) lower_tile(t, ti, tj);
+            tr[h] = c0 + 16 + 16 * ti;
+            tc[h] = c0 + 16 + 16 * tj;
+            acc[h] = dvec4{0.0, 0.0, 0.0, 0.0};
+            if (t < nt) {
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const int kc = c0 + 4 * s + (l >> 4);
-                        acc[h] = mfma_f64(S[tc[h] + (l & 15) + kc * LDL], S[tr[h] + (l & 15) + kc * LDL], acc[h]);
-                    }
+                for (int s = 0; s < 4; ++s) {
+                    const int kc = c0 + 4 * s + (l >> 4);
+                    acc[h] = mfma_f64(S[tc[h] + (l & 15) + kc * LDL], S[tr[h] + (l & 15) + kc * LDL], acc[h]);
                 }
             }
+        }
 #pragma unroll
-            for (int h = 0; h < 10; ++h) {
-                if ((w - 1) + 3 * h < nt) {
-                    double old[4];
+        for (int h = 0; h < 7; ++h) {
+            if (w + 4 * h < nt) {
+                double old[4];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) old[r] = S[tr[h] + (l & 15) + (tc[h] + 4 * r + (l >> 4)) * LDL];
+                for (int r = 0; r < 4; ++r) old[r] = S[tr[h] + (l & 15) + (tc[h] + 4 * r + (l >> 4)) * LDL];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int i = tr[h] + (l & 15), j = tc[h] + 4 * r + (l >> 4);
-                        if (i >= j) S[i + j * LDL] = old[r] - acc[h][r];
-                    }
+                for (int r = 0; r < 4; ++r) {
+                    const int i = tr[h] + (l & 15), j = tc[h] + 4 * r + (l >> 4);
+                    if (i >= j) S[i + j * LDL] = old[r] - acc[h][r];
                 }
             }
         }
@@ -280,10 +256,10 @@ __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64
 // The diagonal role publishes even on a failing pivot, so that no waiter is left behind;
 // waiters then see `info` and stop.  (LLT.h:350-366: the update of block column k by all
 // earlier panels precedes its unblocked factor and the right-side solve.)
-// smem: kSmem doubles of LDS; s_fail/s_ok: LDS ints.  T/PREC: storage and update arithmetic
+// smem: kSmem doubles of LDS; s_ok: an LDS int.  T/PREC: storage and update arithmetic
 // (the diagonal factor and the panel solve are FP64 for every T).
 template <typename T, int PREC>
-__device__ __forceinline__ void panel_role(int g, double *smem, int *s_fail, int *s_ok, T *__restrict__ A,
+__device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__restrict__ A,
                                            int64_t lda, int64_t n, int64_t k0, double *__restrict__ Wleaf, int *info,
                                            uint32_t *flag, uint32_t epoch, int *status)
 {
@@ -341,7 +317,7 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_fail, int
     }
 
     if (g == 0) {
-        const int f = potrf_in_lds(smem, Xl, s_fail);  // begins with a barrier
+        const int f = potrf_in_lds(smem, Xl);  // begins with a barrier
         if (f >= 0) {
             if (tid == 0) atomicCAS(info, 0, (int)(k0 + f + 1));
         } else {

@@ -19,7 +19,10 @@
 //
 // Contributions are accumulated as soon as their block is published (left-looking per row
 // block): 8 waves, one 16-row MFMA tile each, K = 128 per contributing block, Y_j staged
-// through LDS (it is the shared operand of all 8 waves).
+// through LDS (it is the shared operand of all 8 waves) as [rhs + 16 * row], so that the
+// 64 lanes of an MFMA operand read 64 consecutive doubles (no bank conflicts).  Barriers
+// that only protect LDS wait for LDS alone, so the prefetch of the next operand stays in
+// flight across them.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -35,6 +38,14 @@ constexpr uint32_t kSpinLimit = 1u << 22;  // ~0.2 s of polling
 // acquire on every poll invalidates the XCD's L2 each time, and with hundreds of polling
 // workgroups that starves every other memory access.)  Gives up, recording
 // MI_SOLVE_TIMEOUT, after kSpinLimit polls, or at once when another workgroup gave up.
+// Workgroup barrier ordering LDS accesses only (outstanding global loads stay in flight).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ bool wait_flag(const uint32_t *flag, uint32_t epoch, int *status)
 {
     uint32_t spins = 0;
@@ -61,8 +72,8 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
                                                           int *ticket, uint32_t epoch, const int *info, int *status)
 {
     if (factor_failed(info)) return;  // uniform: info is final before this kernel starts
-    __shared__ double Ys[SB * SC];    // Ys[k + SB*c]: the published block being consumed
-    __shared__ double Ts[SB * SC];    // the block's residual before the inverse is applied
+    __shared__ double Ys[SB * SC];    // Ys[c + SC*k]: the published block being consumed
+    __shared__ double Ts[SB * SC];    // Ts[c + SC*r]: the block's residual before the inverse
     __shared__ int s_item;
     __shared__ int s_ok;
     const int nblk = (int)((n + SB - 1) / SB);
@@ -91,25 +102,18 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
     const double *Xp = Winv + (size_t)ib * 2 * SB * SB + (BACKWARD ? 0 : SB * SB);
     const int jstep = BACKWARD ? -1 : 1;
     double pv[SB / 4];
-    // one pointer walked by a fixed stride: consecutive k-steps are 4 rows (backward) or 4
-    // columns (forward) apart; only a last, partial block needs clamped k
-    const int64_t pstep = BACKWARD ? 4 : 4 * lda;
 #define MI_LOAD_OPERAND(jb_)                                                                       \
     do {                                                                                           \
         if ((jb_) != ib) {                                                                         \
             const int64_t j0_ = (int64_t)(jb_)*SB;                                                 \
             const int bj_ = (int)((n - j0_) < SB ? (n - j0_) : SB);                                \
-            if (bj_ == SB) {                                                                       \
-                const T *p_ = BACKWARD ? L + (j0_ + kl) + (i0 + rc) * lda                          \
-                                       : L + (i0 + rc) + (j0_ + kl) * lda;                         \
-                _Pragma("unroll") for (int s = 0; s < SB / 4; ++s) { pv[s] = (double)*p_; p_ += pstep; } \
-            } else {                                                                               \
-                _Pragma("unroll") for (int s = 0; s < SB / 4; ++s)                                 \
-                {                                                                                  \
-                    int k = 4 * s + kl;                                                            \
-                    k = k < bj_ ? k : bj_ - 1;                                                     \
-                    pv[s] = (double)(BACKWARD ? L[(j0_ + k) + (i0 + rc) * lda] : L[(i0 + rc) + (j0_ + k) * lda]); \
-                }                                                                                  \
+            /* uniform part of the address per step (scalar), lane part once; k clamped into a \
+               partial block */                                                                \
+            const T *colbase_ = BACKWARD ? L + j0_ + (i0 + rc) * lda : L + (i0 + rc) + j0_ * lda;   \
+            _Pragma("unroll") for (int s = 0; s < SB / 4; ++s)                                     \
+            {                                                                                      \
+                const int k_ = (4 * s + kl) < bj_ ? (4 * s + kl) : bj_ - 1;                         \
+                pv[s] = (double)colbase_[BACKWARD ? (int64_t)k_ : (int64_t)k_ * lda];              \
             }                                                                                      \
         } else {                                                                                   \
             const double *p_ = Xp + r + SB * kl;                                                   \
@@ -127,18 +131,18 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         const int64_t j0 = (int64_t)jb * SB;
         const int bj = (int)((n - j0) < SB ? (n - j0) : SB);
         for (int e = tid; e < SB * SC; e += 512) {  // Y_j (bj x nc, zero-padded to 128 x 16)
-            const int k = e & (SB - 1), c = e >> 7;
+            const int c = e & (SC - 1), k = e >> 4;
             const bool in = k < bj && c < nc;
             const double v = B[(j0 + (in ? k : 0)) + (c0 + (in ? c : 0)) * ldb];
             Ys[e] = in ? v : 0.0;
         }
-        __syncthreads();
+        lds_barrier();
         // acc[row][c] += sum_k P[row][k] * Ys[k][c]; rows k >= bj of Ys are zero, so the
         // clamped (finite) P values past a partial block contribute nothing
 #pragma unroll
-        for (int s = 0; s < SB / 4; ++s) acc = mfma_f64(Ys[4 * s + kl + SB * (l & 15)], pv[s], acc);
+        for (int s = 0; s < SB / 4; ++s) acc = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], acc);
         MI_LOAD_OPERAND(jb + jstep);  // next block (or op(X_i)): in flight during the next wait
-        __syncthreads();              // Ys is rewritten by the next contribution
+        lds_barrier();                // Ys is rewritten by the next contribution
     }
 #undef MI_LOAD_OPERAND
 
@@ -148,13 +152,13 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         const int c = 4 * e + kl;
         const bool in = r < bi && c < nc;
         const double bv = B[(i0 + rc) + (c0 + (c < nc ? c : 0)) * ldb];
-        Ts[r + SB * c] = in ? bv - acc[e] : 0.0;
+        Ts[c + SC * r] = in ? bv - acc[e] : 0.0;
     }
-    __syncthreads();
+    lds_barrier();
     // Y_i = op(X_i) T
     dvec4 y = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < SB / 4; ++s) y = mfma_f64(Ts[4 * s + kl + SB * (l & 15)], pv[s], y);
+    for (int s = 0; s < SB / 4; ++s) y = mfma_f64(Ts[(l & 15) + SC * (4 * s + kl)], pv[s], y);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int c = 4 * e + kl;
