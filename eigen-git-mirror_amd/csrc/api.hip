@@ -163,9 +163,10 @@ int reserve_sync(mi_llt_handle h, size_t flag_words)
 //
 // Right-looking blocked Cholesky (llt_inplace::blocked, LLT.h:338-368) with a one-panel
 // lookahead, one launch per step on the caller's stream:
-//   launch 0   : panel 0 (diagonal factor + panel solve of block column 0)
-//   launch k+1 : panel k+1 (its update by panel k, diagonal factor, panel solve)
-//                + trailing update by panel k of the columns from (k+2)*NB on.
+//   launch -1 : panel 0 (diagonal factor + panel solve of block column 0)
+//   launch k  : panel k+1 (its update by panel k, diagonal factor, panel solve)
+//               + trailing update by panels k-1 and k (depth 256) of every other block
+//               column from k+2 on (factor_step.hip explains the alternation).
 // The two halves of a launch touch disjoint data; everything they read was finished by an
 // earlier launch, so stream order is the only synchronisation the host needs.
 double panel_flops(int64_t n, int64_t k0, int64_t NB)
@@ -191,14 +192,20 @@ int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, i
     hipStream_t s = h->stream;
     for (int64_t k = -1; k < nblk; ++k) {
         const int64_t kp = (k + 1 < nblk) ? (k + 1) * NB : -1;  // panel of this launch
-        const int64_t ks = (k >= 0 && n - k * NB - 2 * NB > 0) ? k * NB : -1;  // trailing update
-        if (kp < 0 && ks < 0) break;
-        const double m2 = ks >= 0 ? (double)(n - ks - 2 * NB) : 0.0;
-        const double fl = (kp >= 0 ? panel_flops(n, kp, NB) : 0.0) + m2 * (m2 + 1) * NB;
+        // trailing update of launch k: panels k-1, k (k >= 1; panel 0 alone for k = 0) to the
+        // block columns c = k+2, k+4, ... (all rows from c down)
+        const double K = k >= 1 ? 2.0 * NB : (double)NB;
+        double fl_upd = 0.0;
+        for (int64_t c = k + 2; k >= 0 && c < nblk; c += 2) {
+            const double m = (double)(n - c * NB), w = m < NB ? m : (double)NB;
+            fl_upd += 2.0 * K * (m * w - w * (w - 1) / 2);
+        }
+        if (kp < 0 && fl_upd == 0.0) break;
+        const double fl = (kp >= 0 ? panel_flops(n, kp, NB) : 0.0) + fl_upd;
         const uint32_t epoch = (kp >= 0) ? ++h->panel_epoch : h->panel_epoch;
         uint32_t grid = 0;
         traced(h, s, MI_TRACE_POTRF, fl, [&] {
-            mi::launch_factor_step<T>(A, lda, n, kp, ks, wleaf, d_info, h->sync + 4, h->panel_tickets, h->sync + 5,
+            mi::launch_factor_step<T>(A, lda, n, kp, k, wleaf, d_info, h->sync + 4, h->panel_tickets, h->sync + 5,
                                       epoch, (int *)h->sync, s, &grid, prec);
         });
         h->panel_tickets += grid;

@@ -40,6 +40,31 @@ __device__ __forceinline__ fvec4 mfma_bf16(bf16x8 q, bf16x8 p, fvec4 c)
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// Workgroup barrier after which the global stores of every wave are complete (in L2): each
+// wave drains its vector-memory counter (s_waitcnt 0) before the barrier.  Needed (1) before
+// waves read global data that other waves of the workgroup stored, and (2) before one thread
+// publishes the workgroup's stores to other workgroups with an agent-scope release, whose L2
+// writeback then covers all of them.  A plain __syncthreads() orders LDS only here: it does
+// not wait for outstanding global stores.
+__device__ __forceinline__ void global_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Load of data that another workgroup published (a flag release after its stores): an
+// agent-scope atomic load is served from the coherence point, so it cannot hit a stale line
+// in this CU's L1 or this XCD's L2.  (One thread's acquire after the flag invalidates the
+// caches, but the other waves of the workgroup, which only pass a barrier, are not ordered
+// after that invalidation.)
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Early-exit test used by every kernel of a factorization: a previous kernel on the
 // stream recorded a failing pivot.  Uniform across the workgroup.
 __device__ __forceinline__ bool factor_failed(const int *info)

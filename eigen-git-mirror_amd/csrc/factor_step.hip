@@ -1,13 +1,23 @@
 // factor_step.hip -- one launch per block step of the factorization: the panel of block
-// column k+1 (its update by panel k, diagonal factor, panel solve) together with the
-// trailing update by panel k of the columns from k+2 on.
+// column k+1 (its update by panel k, diagonal factor, panel solve) together with a trailing
+// update by the two most recent panels.
 //
 // This is llt_inplace<double,Lower>::blocked (Eigen/src/Cholesky/LLT.h:338-368) with a
 // one-panel lookahead: the two halves of a step touch disjoint data, and everything each
 // depends on was finished by an earlier launch on the same stream, so a single stream and a
-// single launch per step carry all the ordering (no events, no second stream).  Workgroups
-// take roles from a ticket counter, in the order: the diagonal role of the panel (the
-// critical path, always running before anyone waits for it), the trailing-update tiles,
+// single launch per step carry all the ordering (no events, no second stream).
+//
+// Trailing updates two panels deep.  Launch k (panel k just finished) applies the pair of
+// panels k-1, k -- a contiguous 256-column slab -- with depth K = 256 to every other block
+// column c >= k+2, those with c - k even; launch k+1 applies the pair k, k+1 to the others.
+// So every panel reaches every later block column exactly once (panel q through launch q
+// when c - q is even, through launch q+1 when it is odd, and block column q+1 through the
+// panel roles of launch q), each launch does the flops of a one-panel update, and the
+// trailing matrix is read and written half as often as with K = 128 -- at K = 128 that
+// traffic, not the MFMA, bounds the update.  Launch 0 applies panel 0 alone (K = 128).
+//
+// Workgroups take roles from a ticket counter, in the order: the diagonal role of the panel
+// (the critical path, always running before anyone waits for it), the trailing-update tiles,
 // then the panel's row roles -- which wait for the diagonal block, so they should not hold
 // CUs while it is being factored.
 #include "potrf_device.hpp"
@@ -15,16 +25,29 @@
 
 namespace mi {
 
+// Tile u of the update -> (block column c = c0 + 2m, block row i >= c).
+__device__ __forceinline__ void pair_tile(int u, int c0, int nblk, int &i, int &c)
+{
+    int m = 0;
+    while (u >= nblk - c0 - 2 * m) {
+        u -= nblk - c0 - 2 * m;
+        ++m;
+    }
+    c = c0 + 2 * m;
+    i = c + u;
+}
+
 // kp: block column (first column) of the panel to factor, or -1 when none; Gp its roles.
-// ks: first column of the panel whose trailing update runs here, or -1 when none; the
-//     update covers rows/columns from r2 = ks + 256 on, T x T tiles (lower triangle).
+// c0: first block column of the trailing update (every other one from there), nsyrk its
+//     tiles (0: none); slab: first column of the K-wide operand (the last one or two panels).
 // S: storage type (double, or float for the mixed-precision factorization), PREC: the
 // trailing-update arithmetic (kPrecF64 with double; FP32/BF16/split BF16 MFMA with float).
-template <typename S, int PREC>
-__global__ __launch_bounds__(256) void factor_step_kernel(S *__restrict__ A, int64_t lda, int64_t n, int64_t kp,
-                                                          int Gp, int64_t ks, int T, double *__restrict__ Wleaf,
-                                                          int *info, uint32_t *tickets, uint32_t ticket_base,
-                                                          uint32_t *flag, uint32_t epoch, int *status)
+template <typename S, int PREC, int K>
+__global__ __launch_bounds__(256, 2) void factor_step_kernel(S *__restrict__ A, int64_t lda, int64_t n, int64_t kp,
+                                                          int Gp, int c0, int nsyrk, int64_t slab,
+                                                          double *__restrict__ Wleaf, int *info, uint32_t *tickets,
+                                                          uint32_t ticket_base, uint32_t *flag, uint32_t epoch,
+                                                          int *status)
 {
     __shared__ double smem[kSmem];
     __shared__ int s_role, s_ok;
@@ -33,43 +56,72 @@ __global__ __launch_bounds__(256) void factor_step_kernel(S *__restrict__ A, int
     __syncthreads();
     if (factor_failed(info)) return;
     const int t = s_role;
-    const int nsyrk = T * (T + 1) / 2;
     if (Gp > 0 && (t == 0 || t > nsyrk)) {  // ticket 0: diagonal role; last Gp-1: row roles
         panel_role<S, PREC>(t == 0 ? 0 : t - nsyrk, smem, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch,
                             status);
         return;
     }
     const int u = Gp > 0 ? t - 1 : t;
-    const int64_t r2 = ks + 256;
-    int ti, tj;
-    lower_tile(xcd_remap(u, nsyrk), ti, tj);
-    syrk_tile<kNB, S, PREC>(A + r2 + r2 * lda, lda, A + r2 + ks * lda, lda, (int)(n - r2), (int)(n - r2), ti, tj);
+    const int nblk = (int)((n + kNB - 1) / kNB);
+    int i, c;
+    pair_tile(xcd_remap(u, nsyrk), c0, nblk, i, c);
+    const int64_t r = (int64_t)c * kNB;
+    const int64_t m = n - r;
+    syrk_tile<K, S, PREC>(A + r + r * lda, lda, A + r + slab * lda, lda, (int)m, (int)(m < kNB ? m : kNB), i - c,
+                          0);
+}
+
+// Trailing-update tiles of launch k (panel k just finished; k = -1: none).
+int pair_tiles(int64_t n, int64_t k, int *c0_out)
+{
+    const int nblk = (int)((n + kNB - 1) / kNB);
+    const int c0 = (int)k + 2;
+    *c0_out = c0;
+    if (k < 0) return 0;
+    int tiles = 0;
+    for (int c = c0; c < nblk; c += 2) tiles += nblk - c;
+    return tiles;
+}
+
+template <typename S, int PREC>
+static void launch_prec(dim3 grid, hipStream_t s, int K, S *A, int64_t lda, int64_t n, int64_t kp, int Gp, int c0,
+                        int nsyrk, int64_t slab, double *Wleaf, int *info, uint32_t *tickets, uint32_t ticket_base,
+                        uint32_t *flag, uint32_t epoch, int *status)
+{
+    if (K == 256)
+        hipLaunchKernelGGL((factor_step_kernel<S, PREC, 256>), grid, dim3(256), 0, s, A, lda, n, kp, Gp, c0, nsyrk,
+                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status);
+    else
+        hipLaunchKernelGGL((factor_step_kernel<S, PREC, 128>), grid, dim3(256), 0, s, A, lda, n, kp, Gp, c0, nsyrk,
+                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status);
 }
 
 template <typename S>
-void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t ks, double *Wleaf, int *info,
+void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
                         hipStream_t s, uint32_t *grid_out, int prec)
 {
     const int Gp = kp >= 0 ? (int)((n - kp + 127) / 128) : 0;
-    const int64_t m2 = ks >= 0 ? n - ks - 256 : 0;
-    const int T = m2 > 0 ? (int)((m2 + 127) / 128) : 0;
-    const unsigned grid = (unsigned)(Gp + T * (T + 1) / 2);
+    int c0 = 0;
+    const int nsyrk = pair_tiles(n, k, &c0);
+    const int K = k >= 1 ? 256 : 128;
+    const int64_t slab = k >= 1 ? (k - 1) * kNB : 0;
+    const unsigned grid = (unsigned)(Gp + nsyrk);
     *grid_out = grid;
     if (grid == 0) return;
-    const int64_t ksx = m2 > 0 ? ks : (int64_t)-1;
+    const dim3 g(grid);
     if constexpr (sizeof(S) == 8) {
-        hipLaunchKernelGGL((factor_step_kernel<S, kPrecF64>), dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, ksx, T,
-                           Wleaf, info, tickets, ticket_base, flag, epoch, status);
+        launch_prec<S, kPrecF64>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base, flag,
+                                 epoch, status);
     } else if (prec == kPrecBF16) {
-        hipLaunchKernelGGL((factor_step_kernel<S, kPrecBF16>), dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, ksx, T,
-                           Wleaf, info, tickets, ticket_base, flag, epoch, status);
+        launch_prec<S, kPrecBF16>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
+                                  flag, epoch, status);
     } else if (prec == kPrecBF16x3) {
-        hipLaunchKernelGGL((factor_step_kernel<S, kPrecBF16x3>), dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, ksx,
-                           T, Wleaf, info, tickets, ticket_base, flag, epoch, status);
+        launch_prec<S, kPrecBF16x3>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
+                                    flag, epoch, status);
     } else {
-        hipLaunchKernelGGL((factor_step_kernel<S, kPrecF32>), dim3(grid), dim3(256), 0, s, A, lda, n, kp, Gp, ksx, T,
-                           Wleaf, info, tickets, ticket_base, flag, epoch, status);
+        launch_prec<S, kPrecF32>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base, flag,
+                                 epoch, status);
     }
 }
 

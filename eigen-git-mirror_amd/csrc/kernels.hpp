@@ -19,14 +19,14 @@ constexpr int64_t kInvDoubles = 2 * 128 * 128;  // X_b^T then X_b, each 128 x 12
 constexpr int MI_SOLVE_TIMEOUT = 1;
 constexpr int MI_PANEL_TIMEOUT = 2;
 
-// One factorization step: panel of block column kp (or none, kp < 0) + trailing update by
-// the panel at column ks of the columns from ks+256 on (or none, ks < 0).  Writes the grid
+// Launch k of the factorization: panel of block column kp (or none, kp < 0) + the trailing
+// update by panels k-1, k of the block columns k+2, k+4, ... (none for k < 0).  Writes the grid
 // size (= tickets drawn) to *grid_out.  tickets/flag: persistent words of the handle,
 // ticket_base = number of tickets drawn before this launch, epoch = this step's flag value.
 // S = double: FP64 throughout (prec ignored); S = float: FP32 storage, FP64 diagonal factor
 // and panel solve, trailing updates in the MFMA arithmetic `prec` (kPrecF32/BF16/BF16x3).
 template <typename S>
-void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t ks, double *Wleaf, int *info,
+void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
                         hipStream_t s, uint32_t *grid_out, int prec);
 // Winv[b] = {X_b^T, X_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).

@@ -130,35 +130,42 @@ struct WaveTile {
     }
 
     // C[i][j] -= acc (or = acc when ASSIGN) for the elements with i < imax, j < jmax and,
-    // when LOWER, j <= i; i = row0 + row_of(a), j = col0 + col_of(b, r).  All loads of C are
-    // issued before any store, so they overlap.
+    // when LOWER, j <= i; i = row0 + row_of(a), j = col0 + col_of(b, r).  In groups of HA
+    // tile rows, all loads of C are issued before any store, so they overlap (two groups keep
+    // the old values within the register budget of two waves per SIMD).
     template <bool ASSIGN, bool LOWER>
     __device__ __forceinline__ void store(double *C, int64_t ldc, int row0, int col0, int imax, int jmax) const
     {
-        double old[TA][TB][4];
-        if (!ASSIGN) {  // unconditional loads from clamped (in-bounds) addresses
+        constexpr int HA = TA > 1 ? TA / 2 : 1;
 #pragma unroll
-            for (int a = 0; a < TA; ++a)
+        for (int a0 = 0; a0 < TA; a0 += HA) {
+            double old[HA][TB][4];
+            if (!ASSIGN) {  // unconditional loads from clamped (in-bounds) addresses
+#pragma unroll
+                for (int a = 0; a < HA; ++a)
+#pragma unroll
+                    for (int b = 0; b < TB; ++b)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            int i = row0 + row_of(a0 + a), j = col0 + col_of(b, r);
+                            i = i < imax ? i : imax - 1;
+                            j = j < jmax ? j : jmax - 1;
+                            old[a][b][r] = C[i + (int64_t)j * ldc];
+                        }
+            }
+#pragma unroll
+            for (int a = 0; a < HA; ++a)
 #pragma unroll
                 for (int b = 0; b < TB; ++b)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        int i = row0 + row_of(a), j = col0 + col_of(b, r);
-                        i = i < imax ? i : imax - 1;
-                        j = j < jmax ? j : jmax - 1;
-                        old[a][b][r] = C[i + (int64_t)j * ldc];
+                        const int i = row0 + row_of(a0 + a), j = col0 + col_of(b, r);
+                        const bool ok = i < imax && j < jmax && (!LOWER || j <= i);
+                        if (ok)
+                            C[i + (int64_t)j * ldc] =
+                                ASSIGN ? acc[a0 + a][b][r] : old[a][b][r] - acc[a0 + a][b][r];
                     }
         }
-#pragma unroll
-        for (int a = 0; a < TA; ++a)
-#pragma unroll
-            for (int b = 0; b < TB; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = row0 + row_of(a), j = col0 + col_of(b, r);
-                    const bool ok = i < imax && j < jmax && (!LOWER || j <= i);
-                    if (ok) C[i + (int64_t)j * ldc] = ASSIGN ? acc[a][b][r] : old[a][b][r] - acc[a][b][r];
-                }
     }
 
     // Element r (0..3) of tile (a, b) held by this lane is C[i][j] with

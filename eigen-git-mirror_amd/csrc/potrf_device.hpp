@@ -8,7 +8,23 @@
 
 namespace mi {
 
-constexpr int LDL = 144;  // LDS leading dimension: rows 16 apart land in opposite bank halves
+// The 128 x 128 diagonal block lives in LDS as its 36 lower 16 x 16 tiles (72 KiB instead of a
+// full square), each column-major, tile (ti, tj) at 256 * (ti(ti+1)/2 + tj): so two
+// workgroups fit on a CU.  sl(r, c) addresses element (r, c) for r >= c, or any element of a
+// diagonal tile; an MFMA operand (16 rows of 4 consecutive columns) is then 64 consecutive
+// doubles.
+__device__ __forceinline__ int sl(int r, int c)
+{
+    const int ti = r >> 4, tj = c >> 4;
+    return 256 * ((ti * (ti + 1)) / 2 + tj) + (r & 15) + 16 * (c & 15);
+}
+// The strictly lower tiles only (the panel solve keeps L11 without its diagonal tiles, whose
+// inverses it uses instead): r >> 4 > c >> 4.
+__device__ __forceinline__ int sls(int r, int c)
+{
+    const int ti = r >> 4, tj = c >> 4;
+    return 256 * ((ti * (ti - 1)) / 2 + tj) + (r & 15) + 16 * (c & 15);
+}
 
 __device__ __forceinline__ double readlane_d(double x, int lane)
 {
@@ -18,7 +34,7 @@ __device__ __forceinline__ double readlane_d(double x, int lane)
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Panel p (columns c0..c0+15 of the 128 x 128 block in S, ld LDL), all four waves at once.
+// Panel p (columns c0..c0+15 of the 128 x 128 block in S, tiled: sl), all four waves at once.
 // Every wave holds the 16 x 16 leaf in lanes 0..15 (row c0+l) and runs the column loop of
 // LLT.h:320-334 on it (pivot test, sqrt, scale, rank-1 update) redundantly, so its lane
 // broadcasts (readlane) need no barrier.  Lanes 16..63 of waves 0..2 hold the 48 rows
@@ -26,8 +42,10 @@ __device__ __forceinline__ double readlane_d(double x, int lane)
 // solve A_ip L_pp^{-T}; lanes 16..31 of wave 3 hold the rows of the 16 x 16 identity, which the
 // same operations turn into L_pp^{-T}, i.e. column i of X_pp = L_pp^{-1} in lane 16+i, written
 // to xleaf column-major.  1/sqrt from the hardware estimate and one Newton step, d = x/sqrt(x).
-// Returns the failing panel column or -1, the same in every wave.
-__device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restrict__ xleaf)
+// Returns the failing panel column or -1, the same in every wave.  The factored leaf is left
+// in v (lanes 0..15): the caller writes it back to S only after a barrier, since the other
+// waves read the unfactored leaf from S here and may still be doing so when this wave is done.
+__device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restrict__ xleaf, double (&v)[16])
 {
     const int l = lane_id(), w = threadIdx.x >> 6;
     const bool leaf = l < 16;
@@ -35,10 +53,9 @@ __device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restri
     const bool below = !leaf && w < 3 && rb < 128;
     const bool ident = !leaf && w == 3 && l < 32;
     const int r = leaf ? c0 + l : (below ? rb : c0);
-    double v[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
-        const double a = S[r + (c0 + c) * LDL];
+        const double a = S[sl(r, c0 + c)];
         v[c] = leaf ? (c <= l ? a : 0.0) : (below ? a : ((ident && c == l - 16) ? 1.0 : 0.0));
     }
     int fail = -1;
@@ -57,14 +74,9 @@ __device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restri
         }
     }
     if (fail >= 0) return fail;
-    if (leaf && w == 0) {
+    if (below) {  // rows no other wave reads before the next barrier
 #pragma unroll
-        for (int c = 0; c < 16; ++c)
-            if (c <= l) S[r + (c0 + c) * LDL] = v[c];
-    }
-    if (below) {
-#pragma unroll
-        for (int c = 0; c < 16; ++c) S[r + (c0 + c) * LDL] = v[c];
+        for (int c = 0; c < 16; ++c) S[sl(r, c0 + c)] = v[c];
     }
     if (ident) {
 #pragma unroll
@@ -73,10 +85,9 @@ __device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restri
     return -1;
 }
 
-// LDS of one workgroup, shared by the two roles: the 128 x 128 block (ld LDL) for the
-// factor, or L11 (ld 129) plus the 8 leaf inverses for the panel solve.
-constexpr int kSmem = 128 * 129 + 8 * 256;
-static_assert(kSmem >= 128 * LDL, "LDS buffer too small for the block");
+// LDS of one workgroup, shared by the two roles: the 36 lower tiles of the block for the
+// factor, or the 28 strictly lower tiles of L11 plus the 8 leaf inverses for the panel solve.
+constexpr int kSmem = 36 * 256;
 
 // A (global, ld lda, valid order bs) -> S (lower part, identity beyond bs); 32 loads in
 // flight per thread, from clamped in-bounds addresses.  T = float widens to FP64.
@@ -97,7 +108,7 @@ __device__ __forceinline__ void block_to_lds(double *S, const T *__restrict__ Ab
         for (int q = 0; q < 32; ++q) {
             const int c = 2 * (q0 + q) + ch;
             const bool in = r < bs && c < bs;
-            if (r >= c) S[r + c * LDL] = in ? (double)a[q] : (r == c ? 1.0 : 0.0);
+            if (r >= c) S[sl(r, c)] = in ? (double)a[q] : (r == c ? 1.0 : 0.0);
         }
     }
 }
@@ -109,7 +120,7 @@ __device__ __forceinline__ void lds_to_block(const double *S, T *__restrict__ Ab
 #pragma unroll 8
     for (int q = 0; q < 64; ++q) {
         const int c = 2 * q + ch;
-        if (r >= c && r < bs) Ab[r + (int64_t)c * lda] = (T)S[r + c * LDL];
+        if (r >= c && r < bs) Ab[r + (int64_t)c * lda] = (T)S[sl(r, c)];
     }
 }
 
@@ -124,9 +135,15 @@ __device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
     __syncthreads();
     for (int p = 0; p < 8; ++p) {
         const int c0 = 16 * p;
-        const int f = panel_factor4(S, c0, Xl + p * 256);
+        double lv[16];
+        const int f = panel_factor4(S, c0, Xl + p * 256, lv);
         if (f >= 0) return c0 + f;
-        __syncthreads();  // the panel's columns are in S
+        __syncthreads();  // every wave has read the leaf; the rows below it are in S
+        if (w == 0 && l < 16) {  // the leaf itself: read by nobody until the final copy out
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c <= l) S[sl(c0 + l, c0 + c)] = lv[c];
+        }
         const int T = 7 - p;
         const int nt = T * (T + 1) / 2;
         dvec4 acc[7];
@@ -143,7 +160,7 @@ __device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int kc = c0 + 4 * s + (l >> 4);
-                    acc[h] = mfma_f64(S[tc[h] + (l & 15) + kc * LDL], S[tr[h] + (l & 15) + kc * LDL], acc[h]);
+                    acc[h] = mfma_f64(S[sl(tc[h] + (l & 15), kc)], S[sl(tr[h] + (l & 15), kc)], acc[h]);
                 }
             }
         }
@@ -152,11 +169,11 @@ __device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
             if (w + 4 * h < nt) {
                 double old[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) old[r] = S[tr[h] + (l & 15) + (tc[h] + 4 * r + (l >> 4)) * LDL];
+                for (int r = 0; r < 4; ++r) old[r] = S[sl(tr[h] + (l & 15), tc[h] + 4 * r + (l >> 4))];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int i = tr[h] + (l & 15), j = tc[h] + 4 * r + (l >> 4);
-                    if (i >= j) S[i + j * LDL] = old[r] - acc[h][r];
+                    if (i >= j) S[sl(i, j)] = old[r] - acc[h][r];
                 }
             }
         }
@@ -178,14 +195,17 @@ __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64
     const int l = lane_id(), w = threadIdx.x >> 6;
     const int row0 = 32 * w;
     double *Ls = smem;
-    double *Xs = smem + 128 * 129;
-    for (int e = threadIdx.x; e < 128 * 128; e += 256) {
+    double *Xs = smem + 28 * 256;
+    // L11 and its leaf inverses were published by another workgroup: coherent loads
+    for (int e = threadIdx.x; e < 128 * 128; e += 256) {  // strictly lower tiles of L
         const int r = e & 127, c = e >> 7;
-        const bool in = r < bs && c < bs;
-        const double v = (double)L[(in ? r : 0) + (int64_t)(in ? c : 0) * ldl];
-        Ls[r + 129 * c] = (r >= c) ? (in ? v : (r == c ? 1.0 : 0.0)) : 0.0;
+        if ((r >> 4) > (c >> 4)) {
+            const bool in = r < bs && c < bs;
+            const double v = (double)coherent_load(L + (in ? r : 0) + (int64_t)(in ? c : 0) * ldl);
+            Ls[sls(r, c)] = in ? v : 0.0;
+        }
     }
-    for (int e = threadIdx.x; e < 8 * 256; e += 256) Xs[e] = Xl[e];
+    for (int e = threadIdx.x; e < 8 * 256; e += 256) Xs[e] = coherent_load(Xl + e);
     __syncthreads();
 
     dvec4 sol[2][8];
@@ -212,7 +232,7 @@ __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int li = 16 * p + (l & 15), lk = 16 * q + 4 * s + (l >> 4);
-                const double lv = Ls[li + 129 * lk];
+                const double lv = Ls[sls(li, lk)];
 #pragma unroll
                 for (int t = 0; t < 2; ++t) acc[t] = mfma_f64(-lv, sol[t][q][s], acc[t]);
             }
@@ -283,31 +303,35 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__
         }
         if (g == 0) {
             double *S = smem;
-            if (active) {  // S = A_kk - acc on the lower part, identity beyond bs
-                T old[4][4][4];
+            if (active) {  // S = A_kk - acc on the lower part, identity beyond bs; two halves
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a0 = 0; a0 < 4; a0 += 2) {
+                    T old[2][4][4];
 #pragma unroll
-                    for (int b = 0; b < 4; ++b)
+                    for (int a = 0; a < 2; ++a)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            int i = wr * 64 + t.row_of(a), j = wc * 64 + t.col_of(b, r);
-                            i = i < bs ? i : bs - 1;
-                            j = j < bs ? j : bs - 1;
-                            old[a][b][r] = Ck[i + (int64_t)j * lda];
-                        }
+                        for (int b = 0; b < 4; ++b)
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int i = wr * 64 + t.row_of(a), j = wc * 64 + t.col_of(b, r);
-                            if (i >= j) {
-                                const bool in = i < bs && j < bs;
-                                S[i + j * LDL] = in ? (double)old[a][b][r] - (double)t.acc[a][b][r] : (i == j ? 1.0 : 0.0);
+                            for (int r = 0; r < 4; ++r) {
+                                int i = wr * 64 + t.row_of(a0 + a), j = wc * 64 + t.col_of(b, r);
+                                i = i < bs ? i : bs - 1;
+                                j = j < bs ? j : bs - 1;
+                                old[a][b][r] = Ck[i + (int64_t)j * lda];
                             }
-                        }
+#pragma unroll
+                    for (int a = 0; a < 2; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int i = wr * 64 + t.row_of(a0 + a), j = wc * 64 + t.col_of(b, r);
+                                if (i >= j) {
+                                    const bool in = i < bs && j < bs;
+                                    S[sl(i, j)] = in ? (double)old[a][b][r] - (double)t.acc[a0 + a][b][r]
+                                                     : (i == j ? 1.0 : 0.0);
+                                }
+                            }
+                }
             }
         } else {
             t.template store<false, false>(Ck, lda, wr * 64, wc * 64, rows, 128);
@@ -323,11 +347,11 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__
         } else {
             lds_to_block(smem, Ck, lda, bs);
         }
-        __syncthreads();  // every wave's stores of L11 and the leaves are complete
+        global_barrier();  // every wave's stores of L11 and the leaves have reached L2
         if (tid == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    __syncthreads();  // this role's updated rows are stored before they are read back below
+    global_barrier();  // this role's updated rows are in L2 before other waves read them back
     if (tid == 0) {
         uint32_t spins = 0;
         int ok = 1;

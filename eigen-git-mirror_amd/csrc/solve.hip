@@ -133,7 +133,7 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         for (int e = tid; e < SB * SC; e += 512) {  // Y_j (bj x nc, zero-padded to 128 x 16)
             const int c = e & (SC - 1), k = e >> 4;
             const bool in = k < bj && c < nc;
-            const double v = B[(j0 + (in ? k : 0)) + (c0 + (in ? c : 0)) * ldb];
+            const double v = coherent_load(B + (j0 + (in ? k : 0)) + (c0 + (in ? c : 0)) * ldb);
             Ys[e] = in ? v : 0.0;
         }
         lds_barrier();
@@ -164,9 +164,9 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         const int c = 4 * e + kl;
         if (r < bi && c < nc) B[(i0 + r) + (c0 + c) * ldb] = y[e];
     }
-    // the barrier completes every wave's stores (workgroup release); one agent-scope release
-    // store then publishes them all
-    __syncthreads();
+    // every wave's stores reach L2 before the barrier; one agent-scope release store then
+    // publishes them all
+    global_barrier();
     if (tid == 0) __hip_atomic_store(gflags + ib, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 

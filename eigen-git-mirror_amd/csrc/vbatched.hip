@@ -67,10 +67,12 @@ struct VbArith<float> {
     }
 };
 
-// Memory ordering between the lanes of one wave for data it stored and reads back.
+// Memory ordering between the lanes of one wave for data it stored and reads back: the
+// wave's global stores complete (s_waitcnt 0) before any later load is issued.
 __device__ __forceinline__ void vb_fence()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
