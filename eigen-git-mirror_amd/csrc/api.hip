@@ -125,7 +125,7 @@ struct Layout {
 int prepare(mi_llt_handle h, int64_t n, bool upper, Layout &lay)
 {
     const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
-    const size_t leaf = round_up((size_t)nblk * mi::kLeafDoubles * sizeof(double));
+    const size_t leaf = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double));
     const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
     size_t need = leaf + inv;
     if (upper) need += round_up((size_t)n * n * sizeof(double));
@@ -267,7 +267,7 @@ int prepare_mixed(mi_llt_handle h, int64_t n, int64_t nrhs, bool fp64, MixedLayo
 {
     const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
     const size_t wbytes = round_up((size_t)n * (size_t)n * (fp64 ? sizeof(double) : sizeof(float)));
-    const size_t leaf = round_up((size_t)nblk * mi::kLeafDoubles * sizeof(double));
+    const size_t leaf = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double));
     const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
     const size_t vec = round_up((size_t)n * (size_t)nrhs * sizeof(double));
     const size_t part = round_up((size_t)mi::residual_chunks(n, nrhs) * (size_t)n * (size_t)nrhs * sizeof(double));
@@ -721,7 +721,7 @@ int mi_dpotrf_adjoint(mi_llt_handle h, char uplo, int64_t n, const double *dL, i
     if (n == 0) return 0;
     hipStream_t s = h->stream;
     const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
-    const size_t leaf = round_up((size_t)nblk * mi::kLeafDoubles * sizeof(double));
+    const size_t leaf = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double));
     const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
     const size_t sq = round_up((size_t)n * (size_t)n * sizeof(double));
     int rc = reserve(h, leaf + inv + 2 * sq);
@@ -762,7 +762,7 @@ int mi_dpotrs_adjoint(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const
     if (n == 0 || nrhs == 0) return 0;
     hipStream_t s = h->stream;
     const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
-    const size_t leaf = round_up((size_t)nblk * mi::kLeafDoubles * sizeof(double));
+    const size_t leaf = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double));
     const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
     const size_t vec = round_up((size_t)n * (size_t)nrhs * sizeof(double));
     int rc = reserve(h, leaf + inv + 2 * vec);

@@ -42,20 +42,26 @@ __device__ __forceinline__ void pair_tile(int u, int c0, int nblk, int &i, int &
 //     tiles (0: none); slab: first column of the K-wide operand (the last one or two panels).
 // S: storage type (double, or float for the mixed-precision factorization), PREC: the
 // trailing-update arithmetic (kPrecF64 with double; FP32/BF16/split BF16 MFMA with float).
+// Workgroups per CU the kernel is compiled for: 2 fits two 72 KiB LDS blocks and 256 VGPRs
+// per lane on a CU (the second workgroup hides the first one's barrier and memory waits).
+#ifndef MI_FACTOR_WG_PER_CU
+#define MI_FACTOR_WG_PER_CU 2
+#endif
+
 template <typename S, int PREC, int K>
-__global__ __launch_bounds__(256, 2) void factor_step_kernel(S *__restrict__ A, int64_t lda, int64_t n, int64_t kp,
+__global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S *__restrict__ A, int64_t lda, int64_t n, int64_t kp,
                                                           int Gp, int c0, int nsyrk, int64_t slab,
                                                           double *__restrict__ Wleaf, int *info, uint32_t *tickets,
                                                           uint32_t ticket_base, uint32_t *flag, uint32_t epoch,
                                                           int *status)
 {
-    __shared__ double smem[kSmem];
+    __shared__ __align__(16) double smem[kSmem];
     __shared__ int s_role, s_ok;
     // every workgroup draws its ticket, even when it then stops: the host counts them
     if (threadIdx.x == 0) s_role = (int)(atomicAdd(tickets, 1u) - ticket_base);
     __syncthreads();
     if (factor_failed(info)) return;
-    const int t = s_role;
+    const int t = __builtin_amdgcn_readfirstlane(s_role);  // uniform: every derived address is scalar
     if (Gp > 0 && (t == 0 || t > nsyrk)) {  // ticket 0: diagonal role; last Gp-1: row roles
         panel_role<S, PREC>(t == 0 ? 0 : t - nsyrk, smem, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch,
                             status);

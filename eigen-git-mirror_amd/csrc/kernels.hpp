@@ -10,9 +10,14 @@ namespace mi {
 constexpr int kNB = 128;
 
 // Workspace of one factorization, per 128-block b:
-//   Wleaf[b]: the 8 inverses of the 16x16 diagonal leaves of L_bb (8 x 256 doubles)
+//   Wleaf[b]: the panel image of L_bb, kPanelDoubles: its 28 strictly lower 16 x 16 tiles
+//             (tile (ti, tj), ti > tj, at 256 * (ti(ti-1)/2 + tj), each column-major), then at
+//             kLeafOffset the 8 inverses of its 16 x 16 diagonal leaves (column-major).  The
+//             factorization's diagonal role publishes it for the panel solve, which copies it
+//             to LDS as it is; the solve-only path fills the leaf part alone.
 //   Winv[b] : L_bb^{-T}, 128 x 128, ld 128 (built for the solve)
-constexpr int64_t kLeafDoubles = 8 * 256;
+constexpr int64_t kPanelDoubles = 36 * 256;
+constexpr int64_t kLeafOffset = 28 * 256;
 constexpr int64_t kInvDoubles = 2 * 128 * 128;  // X_b^T then X_b, each 128 x 128, ld 128
 
 // Device status bits (a word in the handle's persistent device memory).

@@ -16,17 +16,25 @@
 
 namespace mi {
 
-// MASKED: k may stop short of a multiple of 4 (kmax % 4 != 0); the mask is applied at
-// MFMA time so that loads stay in flight (a select right after a load would force a wait).
-template <int TA, int TB, int D = 3, bool MASKED = false>
+// Addressing: each operand has a wave-uniform base pointer, advanced by 4 columns per k-step
+// (scalar registers), and a per-lane 32-bit byte offset of (its row, column lane/16), fixed
+// for the whole run -- global loads in the scalar-base + vector-offset form, no per-load
+// address arithmetic.  Requires (row + 3 * ld) * 8 < 4 GiB for both operands.
+// NEG: accumulate -P Q^T (acc starts from C for C - P Q^T, see load_acc).  ACTIVE: bit
+// a*TB+b set for the MFMA tiles computed; the others are never issued (an irregular tile set,
+// e.g. the lower-triangle tiles of two tile rows).  QDENSE: the 16*TB rows of Q are all valid
+// (no clamping), so one offset plus constants serves them (fewer registers).
+template <int TA, int TB, int D = 3, bool NEG = false, uint32_t ACTIVE = 0xffffffffu, bool QDENSE = false>
 struct WaveTile {
+    static_assert(TA * TB <= 32, "ACTIVE has one bit per MFMA tile");
     static constexpr int U = D + 1;  // rolling buffer depth
     static constexpr int KS = 4;     // k per MFMA step
+    __device__ __forceinline__ static constexpr bool on(int a, int b) { return (ACTIVE >> (a * TB + b)) & 1u; }
     dvec4 acc[TA][TB];
-    const double *pa[TA];
-    const double *qb[TB];
+    const double *pbase, *qbase;  // wave-uniform
+    uint32_t po[TA], qo[TB];      // per-lane byte offsets
     int64_t ldp, ldq;
-    int kmax;  // number of valid k
+    int kmax;  // number of valid k (a multiple of 4)
 
     __device__ __forceinline__ void zero()
     {
@@ -36,63 +44,101 @@ struct WaveTile {
             for (int b = 0; b < TB; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
     }
 
+    // P rows p0[a] .. p0[a]+15 for tile row a (valid rows < pmax; the rows of different tile
+    // rows need not be adjacent), Q rows q0 .. q0+16*TB-1 (valid < qmax).
+    __device__ __forceinline__ void setup_rows(const double *P, int64_t ldp_, const int (&p0)[TA], int pmax,
+                                               const double *Q, int64_t ldq_, int q0, int qmax, int kmax_)
+    {
+        const int l = lane_id();
+        pbase = P;
+        qbase = Q;
+        ldp = ldp_;
+        ldq = ldq_;
+        kmax = kmax_;
+        const int kl = l >> 4;
+#pragma unroll
+        for (int a = 0; a < TA; ++a) {
+            int r = p0[a] + (l & 15);
+            r = r < pmax ? r : pmax - 1;
+            po[a] = (uint32_t)((r + kl * ldp_) * 8);
+        }
+        if (QDENSE) {
+            qo[0] = (uint32_t)((q0 + (l & 15) + kl * ldq_) * 8);
+        } else {
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+                int r = q0 + 16 * b + (l & 15);
+                r = r < qmax ? r : qmax - 1;
+                qo[b] = (uint32_t)((r + kl * ldq_) * 8);
+            }
+        }
+    }
+
     // P rows p0 .. p0+16*TA-1 (valid rows < pmax), Q rows q0 .. (valid < qmax).
     __device__ __forceinline__ void setup(const double *P, int64_t ldp_, int p0, int pmax, const double *Q,
                                           int64_t ldq_, int q0, int qmax, int kmax_)
     {
-        const int l = lane_id();
-        ldp = ldp_;
-        ldq = ldq_;
-        kmax = kmax_;
+        int rows[TA];
 #pragma unroll
-        for (int a = 0; a < TA; ++a) {
-            int r = p0 + 16 * a + (l & 15);
-            r = r < pmax ? r : pmax - 1;
-            pa[a] = P + r;
-        }
+        for (int a = 0; a < TA; ++a) rows[a] = p0 + 16 * a;
+        setup_rows(P, ldp_, rows, pmax, Q, ldq_, q0, qmax, kmax_);
+    }
+
+    // acc <- C for the active tiles: tile (a, b) is rows p0[a].., columns col0 + 16b..;
+    // loads from clamped (in-bounds) addresses, i < imax, j < jmax.
+    __device__ __forceinline__ void load_acc(const double *C, int64_t ldc, const int (&p0)[TA], int col0, int imax,
+                                             int jmax)
+    {
 #pragma unroll
-        for (int b = 0; b < TB; ++b) {
-            int r = q0 + 16 * b + (l & 15);
-            r = r < qmax ? r : qmax - 1;
-            qb[b] = Q + r;
-        }
+        for (int a = 0; a < TA; ++a)
+#pragma unroll
+            for (int b = 0; b < TB; ++b)
+                if (on(a, b)) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        int i = p0[a] + (lane_id() & 15), j = col0 + col_of(b, r);
+                        i = i < imax ? i : imax - 1;
+                        j = j < jmax ? j : jmax - 1;
+                        acc[a][b][r] = C[i + (int64_t)j * ldc];
+                    }
+                }
     }
 
     struct Frag {
         double p[TA], q[TB];
     };
 
-    // Raw loads only, always from in-bounds addresses: the step is clamped to the last one
-    // (prefetches past the end re-read it) and, when MASKED, lanes whose k is past kmax read
-    // column 0.  mma() zeroes what must not count.
+    __device__ __forceinline__ static double at(const double *base, uint32_t off)
+    {
+        return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+    }
+
+    // Loads of k-step s (clamped to the last one: prefetches past the end re-read it).
     __device__ __forceinline__ void load(Frag &f, int s, int S) const
     {
         s = s < S ? s : S - 1;
-        int k = 4 * s + (lane_id() >> 4);
-        if (MASKED) k = k < kmax ? k : 0;
-        const int64_t op = (int64_t)k * ldp, oq = (int64_t)k * ldq;
+        const double *pb = pbase + (int64_t)(4 * s) * ldp;
+        const double *qb = qbase + (int64_t)(4 * s) * ldq;
 #pragma unroll
-        for (int a = 0; a < TA; ++a) f.p[a] = pa[a][op];
+        for (int a = 0; a < TA; ++a) f.p[a] = at(pb, po[a]);
 #pragma unroll
-        for (int b = 0; b < TB; ++b) f.q[b] = qb[b][oq];
+        for (int b = 0; b < TB; ++b) f.q[b] = QDENSE ? at(qb + 16 * b, qo[0]) : at(qb, qo[b]);
     }
 
-    __device__ __forceinline__ void mma(const Frag &f, int s)
+    __device__ __forceinline__ void mma(const Frag &f, int /*s*/)
     {
-        bool ok = true;
-        if (MASKED) ok = 4 * s + (lane_id() >> 4) < kmax;
 #pragma unroll
-        for (int a = 0; a < TA; ++a)
+        for (int a = 0; a < TA; ++a) {
+            const double p = NEG ? -f.p[a] : f.p[a];
 #pragma unroll
-            for (int b = 0; b < TB; ++b) {
-                const double q = MASKED ? (ok ? f.q[b] : 0.0) : f.q[b];
-                acc[a][b] = mfma_f64(q, f.p[a], acc[a][b]);
-            }
+            for (int b = 0; b < TB; ++b)
+                if (on(a, b)) acc[a][b] = mfma_f64(f.q[b], p, acc[a][b]);
+        }
     }
 
     // Accumulate over k in [0, kmax), 4 k per MFMA step, D steps of loads in flight.  The
     // loop body has no branches, so the compiler's wait counts stay exact: the step count
-    // is padded to a multiple of U, which requires MASKED unless kmax % (4U) == 0.
+    // is padded to a multiple of U, which requires kmax % (4U) == 0.
     __device__ __forceinline__ void run()
     {
         const int S = (kmax + 3) >> 2;

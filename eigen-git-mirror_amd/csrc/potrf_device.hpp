@@ -42,6 +42,15 @@ __device__ __forceinline__ double readlane_d(double x, int lane)
 // solve A_ip L_pp^{-T}; lanes 16..31 of wave 3 hold the rows of the 16 x 16 identity, which the
 // same operations turn into L_pp^{-T}, i.e. column i of X_pp = L_pp^{-1} in lane 16+i, written
 // to xleaf column-major.  1/sqrt from the hardware estimate and one Newton step, d = x/sqrt(x).
+//
+// The column operations run on every lane without lane predicates: at step j, v[j] *= 1/sqrt(x)
+// gives sqrt(x) on lane j (where v[j] is the pivot x) and the scaled column below it, and the
+// rank-1 update v[c] -= v[j] * L(c0+c, c0+j), c > j, leaves every lane's entries at or below
+// its diagonal exact.  Entries above the diagonal of a leaf lane l (c > l) do change, but they
+// are never broadcast (readlane reads lane c of column j < c) nor stored.  Without per-lane
+// selects the loop holds no predicate masks, so the broadcasts get scalar registers of their
+// own and column j+1 -- the next pivot -- is updated first, off the remaining updates.
+//
 // Returns the failing panel column or -1, the same in every wave.  The factored leaf is left
 // in v (lanes 0..15): the caller writes it back to S only after a barrier, since the other
 // waves read the unfactored leaf from S here and may still be doing so when this wave is done.
@@ -54,24 +63,26 @@ __device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restri
     const bool ident = !leaf && w == 3 && l < 32;
     const int r = leaf ? c0 + l : (below ? rb : c0);
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        const double a = S[sl(r, c0 + c)];
-        v[c] = leaf ? (c <= l ? a : 0.0) : (below ? a : ((ident && c == l - 16) ? 1.0 : 0.0));
+    for (int c = 0; c < 16; ++c) v[c] = S[sl(r, c0 + c)];
+    if (ident) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = (c == l - 16) ? 1.0 : 0.0;
     }
     int fail = -1;
+    double x = readlane_d(v[0], 0);  // pivot: leaf row j lives in lane j
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const double x = readlane_d(v[j], j);  // pivot: leaf row j lives in lane j
-        if (fail < 0 && x <= 0.0) fail = j;    // LLT.h:327-330; a NaN pivot passes
+        if (fail < 0 && x <= 0.0) fail = j;  // LLT.h:327-330; a NaN pivot passes
         double rd = __builtin_amdgcn_rsq(x);
         rd = rd * (1.5 - 0.5 * x * rd * rd);
-        const double d = x * rd;
-        v[j] = l == j ? d : (l > j ? v[j] * rd : v[j]);
-#pragma unroll
-        for (int c = j + 1; c < 16; ++c) {
-            const double lcj = readlane_d(v[j], c);  // L(c0+c, c0+j)
-            v[c] -= v[j] * lcj;
+        v[j] *= rd;
+        if (j + 1 < 16) {  // the next pivot first: it is the loop's serial chain
+            v[j + 1] -= v[j] * readlane_d(v[j], j + 1);
+            x = readlane_d(v[j + 1], j + 1);
+            __builtin_amdgcn_sched_barrier(0);
         }
+#pragma unroll
+        for (int c = j + 2; c < 16; ++c) v[c] -= v[j] * readlane_d(v[j], c);  // L(c0+c, c0+j)
     }
     if (fail >= 0) return fail;
     if (below) {  // rows no other wave reads before the next barrier
@@ -126,12 +137,16 @@ __device__ __forceinline__ void lds_to_block(const double *S, T *__restrict__ Ab
 
 // Cholesky of the 128 x 128 lower block in S (all 256 threads), in eight 16-column panels:
 // panel factor and solve by all four waves (panel_factor4), then the rank-16 update of the
-// rest of the block by the panel on FP64 MFMA from LDS, the 28..1 lower 16 x 16 tiles dealt
-// round-robin to the waves, all MFMAs before the read-modify-writes.  Returns the first
-// failing column or -1 (uniform); writes the 8 leaf inverses to Xl.
+// T = 7-p tile rows below by the panel on FP64 MFMA from LDS.  Every wave first loads the
+// panel's T tile-row operands (lane l, k-step s: row l%16, column 4s + l/16 -- the same layout
+// for both MFMA operands) into registers; the T(T+1)/2 lower tiles are then dealt round-robin
+// to the waves by a wave-uniform test over fully unrolled tile loops.  A diagonal tile is
+// updated whole: the entries above its diagonal are never read as data (the column loop keeps
+// them off its broadcasts and the copy out stores r >= c only).  Returns the first failing
+// column or -1 (uniform); writes the 8 leaf inverses to Xl.
 __device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
 {
-    const int w = threadIdx.x >> 6, l = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = lane_id();
     __syncthreads();
     for (int p = 0; p < 8; ++p) {
         const int c0 = 16 * p;
@@ -141,39 +156,34 @@ __device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
         __syncthreads();  // every wave has read the leaf; the rows below it are in S
         if (w == 0 && l < 16) {  // the leaf itself: read by nobody until the final copy out
 #pragma unroll
-            for (int c = 0; c < 16; ++c)
-                if (c <= l) S[sl(c0 + l, c0 + c)] = lv[c];
+            for (int c = 0; c < 16; ++c) S[sl(c0 + l, c0 + c)] = lv[c];
         }
         const int T = 7 - p;
-        const int nt = T * (T + 1) / 2;
-        dvec4 acc[7];
-        int tr[7], tc[7];
+        const int tb = p + 1;  // first trailing tile row
+        double P[7][4];
 #pragma unroll
-        for (int h = 0; h < 7; ++h) {
-            const int t = w + 4 * h;
-            int ti = 0, tj = 0;
-            if (t < nt) lower_tile(t, ti, tj);
-            tr[h] = c0 + 16 + 16 * ti;
-            tc[h] = c0 + 16 + 16 * tj;
-            acc[h] = dvec4{0.0, 0.0, 0.0, 0.0};
-            if (t < nt) {
+        for (int x = 0; x < 7; ++x) {
+            if (x < T) {
+                const double *src = S + 256 * (((tb + x) * (tb + x + 1)) / 2 + p) + l;
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int kc = c0 + 4 * s + (l >> 4);
-                    acc[h] = mfma_f64(S[sl(tc[h] + (l & 15), kc)], S[sl(tr[h] + (l & 15), kc)], acc[h]);
-                }
+                for (int s = 0; s < 4; ++s) P[x][s] = src[64 * s];
             }
         }
 #pragma unroll
-        for (int h = 0; h < 7; ++h) {
-            if (w + 4 * h < nt) {
-                double old[4];
+        for (int ti = 0; ti < 7; ++ti) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) old[r] = S[sl(tr[h] + (l & 15), tc[h] + 4 * r + (l >> 4))];
+            for (int tj = 0; tj <= ti; ++tj) {
+                const int t = (ti * (ti + 1)) / 2 + tj;
+                if (ti < T && (t & 3) == w) {
+                    double *C = S + 256 * (((tb + ti) * (tb + ti + 1)) / 2 + tb + tj) + l;
+                    double old[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = tr[h] + (l & 15), j = tc[h] + 4 * r + (l >> 4);
-                    if (i >= j) S[sl(i, j)] = old[r] - acc[h][r];
+                    for (int r = 0; r < 4; ++r) old[r] = C[64 * r];
+                    dvec4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) acc = mfma_f64(P[tj][s], P[ti][s], acc);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) C[64 * r] = old[r] - acc[r];
                 }
             }
         }
@@ -182,72 +192,106 @@ __device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
     return -1;
 }
 
-// Rows 0..127 of P (valid m) <- P * L^{-T} with L (128 x 128 lower, ld ldl, valid order bs,
-// identity beyond) and its 8 leaf inverses Xl, using smem for L and the leaves.  Each wave
-// owns 2 row tiles of 16 rows.  IDENTITY: P is the identity (not read); the result L^{-T}
-// goes to O (ld 128) and its transpose L^{-1} to O + 128*128.  T = float: P and L are read
-// as FP32 and widened, the arithmetic is FP64, the result is rounded back to FP32.
-template <bool IDENTITY, typename T>
-__device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64_t ldp, int m,
-                                          const T *__restrict__ L, int64_t ldl, int bs,
-                                          const double *__restrict__ Xl, double *__restrict__ O)
+// The panel solve works on a wave's 32 rows x 128 columns in the FP64 MFMA result layout:
+// X[t][p][r] is row 16t + lane%16, column 16p + 4r + lane/16 (t = 0..1 row tiles, p = 0..7
+// column strips).  It needs, in LDS, the 28 strictly lower tiles of L (Ls, sls order) and the
+// 8 leaf inverses (Xs, 256 doubles each, column-major): the panel image of kernels.hpp.
+//
+// X <- X L^{-T} in place, strip by strip (TriangularSolverMatrix.h:186-329, blocked by 16):
+//   X_p <- (X_p - sum_{q<p} X_q L_pq^T) X_pp^T,   X_pp = L_pp^{-1}
+// with the accumulator of a solved strip used directly as the operand of the next ones (its
+// layout is that of an MFMA P operand with k = 4s + lane/16).
+__device__ __forceinline__ void trsm_in_registers(const double *smem, dvec4 (&X)[2][8])
 {
-    const int l = lane_id(), w = threadIdx.x >> 6;
-    const int row0 = 32 * w;
-    double *Ls = smem;
-    double *Xs = smem + 28 * 256;
-    // L11 and its leaf inverses were published by another workgroup: coherent loads
-    for (int e = threadIdx.x; e < 128 * 128; e += 256) {  // strictly lower tiles of L
-        const int r = e & 127, c = e >> 7;
-        if ((r >> 4) > (c >> 4)) {
-            const bool in = r < bs && c < bs;
-            const double v = (double)coherent_load(L + (in ? r : 0) + (int64_t)(in ? c : 0) * ldl);
-            Ls[sls(r, c)] = in ? v : 0.0;
-        }
-    }
-    for (int e = threadIdx.x; e < 8 * 256; e += 256) Xs[e] = coherent_load(Xl + e);
-    __syncthreads();
-
-    dvec4 sol[2][8];
+    const int l = lane_id();
+    const double *Ls = smem;
+    const double *Xs = smem + kLeafOffset;
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-        dvec4 acc[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int i = row0 + 16 * t + (l & 15);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int j = 16 * p + 4 * r + (l >> 4);
-                if (IDENTITY) {
-                    acc[t][r] = (i == j) ? 1.0 : 0.0;
-                } else {
-                    const double pv = (double)P[(i < m ? i : m - 1) + (int64_t)j * ldp];
-                    acc[t][r] = (i < m) ? pv : 0.0;
-                }
-            }
-        }
-        // acc -= sum_{q<p} sol_q * L[16p.., 16q..]^T
 #pragma unroll
         for (int q = 0; q < p; ++q) {
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const int li = 16 * p + (l & 15), lk = 16 * q + 4 * s + (l >> 4);
-                const double lv = Ls[sls(li, lk)];
+                const double lv = Ls[256 * ((p * (p - 1)) / 2 + q) + l + 64 * s];  // L[16p + l%16][16q + 4s + l/16]
 #pragma unroll
-                for (int t = 0; t < 2; ++t) acc[t] = mfma_f64(-lv, sol[t][q][s], acc[t]);
+                for (int t = 0; t < 2; ++t) X[t][p] = mfma_f64(-lv, X[t][q][s], X[t][p]);
             }
         }
-        // sol_p = acc * X_pp^T
         dvec4 z[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const double xv = Xs[p * 256 + (l & 15) + 16 * (4 * s + (l >> 4))];
+            const double xv = Xs[256 * p + l + 64 * s];  // X_pp[l%16][4s + l/16]
 #pragma unroll
-            for (int t = 0; t < 2; ++t) z[t] = mfma_f64(xv, acc[t][s], z[t]);
+            for (int t = 0; t < 2; ++t) z[t] = mfma_f64(xv, X[t][p][s], z[t]);
         }
-        sol[0][p] = z[0];
-        sol[1][p] = z[1];
+        X[0][p] = z[0];
+        X[1][p] = z[1];
     }
+}
+
+// L (128 x 128 lower, ld ldl, valid order bs, identity beyond; T = float widens) and the leaf
+// inverses Xl -> the panel image in LDS.
+template <typename T>
+__device__ __forceinline__ void stage_factor_image(double *smem, const T *__restrict__ L, int64_t ldl, int bs,
+                                                   const double *__restrict__ Xl)
+{
+    for (int e = threadIdx.x; e < 128 * 128; e += 256) {  // strictly lower tiles of L
+        const int r = e & 127, c = e >> 7;
+        if ((r >> 4) > (c >> 4)) {
+            const bool in = r < bs && c < bs;
+            const double v = (double)L[(in ? r : 0) + (int64_t)(in ? c : 0) * ldl];
+            smem[sls(r, c)] = in ? v : 0.0;
+        }
+    }
+    for (int e = threadIdx.x; e < 8 * 256; e += 256) smem[kLeafOffset + e] = Xl[e];
+}
+
+// A published panel image (kPanelDoubles, 16-byte aligned) -> LDS, 16 bytes per load.
+__device__ __forceinline__ void stage_panel_image(double *smem, const double *__restrict__ W)
+{
+    const double2 *src = reinterpret_cast<const double2 *>(W);
+    double2 *dst = reinterpret_cast<double2 *>(smem);
+#pragma unroll
+    for (int i = 0; i < (int)(kPanelDoubles / 2 / 256); ++i) dst[threadIdx.x + 256 * i] = src[threadIdx.x + 256 * i];
+}
+
+// Rows 0..127 of P (valid m) <- P * L^{-T} with L and its 8 leaf inverses Xl (read from
+// global into the LDS panel image).  Each wave owns 32 rows.  IDENTITY: P is the identity (not
+// read); the result L^{-T} goes to O (ld 128) and its transpose L^{-1} to O + 128*128.  T =
+// float: P and L are read as FP32 and widened, the arithmetic is FP64, the result is rounded
+// back to FP32.  If IMAGE is given, the panel image is copied from there instead of built from
+// L and Xl.
+template <bool IDENTITY, typename T>
+__device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64_t ldp, int m,
+                                          const T *__restrict__ L, int64_t ldl, int bs,
+                                          const double *__restrict__ Xl, double *__restrict__ O,
+                                          const double *__restrict__ image = nullptr)
+{
+    const int l = lane_id(), w = threadIdx.x >> 6;
+    const int row0 = 32 * w;
+    dvec4 X[2][8];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int i = row0 + 16 * t + (l & 15);
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = 16 * p + 4 * r + (l >> 4);
+                if (IDENTITY) {
+                    X[t][p][r] = (i == j) ? 1.0 : 0.0;
+                } else {
+                    const double pv = (double)P[(i < m ? i : m - 1) + (int64_t)j * ldp];
+                    X[t][p][r] = (i < m) ? pv : 0.0;
+                }
+            }
+    }
+    if (image)
+        stage_panel_image(smem, image);
+    else
+        stage_factor_image<T>(smem, L, ldl, bs, Xl);
+    __syncthreads();
+    trsm_in_registers(smem, X);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int i = row0 + 16 * t + (l & 15);
@@ -258,101 +302,107 @@ __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64
                 for (int r = 0; r < 4; ++r) {
                     const int j = 16 * p + 4 * r + (l >> 4);
                     if (IDENTITY) {
-                        O[i + (int64_t)j * 128] = sol[t][p][r];
-                        O[128 * 128 + j + 128 * i] = sol[t][p][r];
+                        O[i + (int64_t)j * 128] = X[t][p][r];
+                        O[128 * 128 + j + 128 * i] = X[t][p][r];
                     } else {
-                        P[i + (int64_t)j * ldp] = (T)sol[t][p][r];
+                        P[i + (int64_t)j * ldp] = (T)X[t][p][r];
                     }
                 }
         }
     }
 }
 
-// The panel role of one block step; g is the role from the ticket:
-//   role 0   : C = A_kk - P_0 P_0^T straight into LDS, Cholesky in LDS, L11 and the leaf
-//              inverses to memory, then publish `flag` = epoch (release);
-//   role g>0 : rows k0+128g..+127: C = A_gk - P_g P_0^T written back, wait for the flag,
-//              then rows <- rows * L11^{-T}.
-// The diagonal role publishes even on a failing pivot, so that no waiter is left behind;
-// waiters then see `info` and stop.  (LLT.h:350-366: the update of block column k by all
-// earlier panels precedes its unblocked factor and the right-side solve.)
-// smem: kSmem doubles of LDS; s_ok: an LDS int.  T/PREC: storage and update arithmetic
-// (the diagonal factor and the panel solve are FP64 for every T).
-template <typename T, int PREC>
-__device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__restrict__ A,
-                                           int64_t lda, int64_t n, int64_t k0, double *__restrict__ Wleaf, int *info,
-                                           uint32_t *flag, uint32_t epoch, int *status)
+// FP64 update of the diagonal block straight into LDS: S = A_kk - P_0 P_0^T for tile rows W
+// and 7-W, their tiles at and left of the diagonal -- 9 tiles per wave, the 36 lower tiles
+// over the 4 waves -- with acc starting from A_kk; identity beyond bs.  A diagonal tile is
+// written whole (its upper entries are never read as data).
+template <int W>
+__device__ __forceinline__ void diag_update_f64(double *S, const double *__restrict__ Akk,
+                                                const double *__restrict__ P0, int64_t lda, int bs)
 {
-    const int tid = threadIdx.x, w = tid >> 6;
-    const int64_t R0 = k0 + 128 * (int64_t)g;
-    const int rows = (int)((n - R0) < 128 ? (n - R0) : 128);
-    const int bs = (int)((n - k0) < 128 ? (n - k0) : 128);
-    T *Ck = A + R0 + k0 * lda;  // this role's 128 rows of block column k0
-    double *Xl = Wleaf + (size_t)(k0 / 128) * kLeafDoubles;
-
-    if (k0 > 0) {
-        // C -= P_g P_0^T, P = previous panel (columns k0-128..k0-1): rows R0.. and k0..
-        const int wr = w >> 1, wc = w & 1;
-        const bool active = !(g == 0 && wr == 0 && wc == 1);  // strictly upper quadrant of the diagonal tile
-        using Tile = typename TileSel<T, PREC>::type;
-        Tile t;
-        t.zero();
-        if (active) {
-            t.setup(A + R0 + (k0 - 128) * lda, lda, wr * 64, rows, A + k0 + (k0 - 128) * lda, lda, wc * 64, bs, 128);
-            t.template run_fixed<128 / Tile::KS>();
-        }
-        if (g == 0) {
-            double *S = smem;
-            if (active) {  // S = A_kk - acc on the lower part, identity beyond bs; two halves
+    constexpr uint32_t kActive = ((1u << (W + 1)) - 1u) | (((1u << (8 - W)) - 1u) << 8);
+    WaveTile<2, 8, 2, true, kActive> t;
+    const int rows[2] = {16 * W, 16 * (7 - W)};
+    t.zero();
+    t.load_acc(Akk, lda, rows, 0, bs, bs);
+    t.setup_rows(P0, lda, rows, bs, P0, lda, 0, bs, 128);
+    t.template run_fixed<32>();
+    const int l = lane_id();
 #pragma unroll
-                for (int a0 = 0; a0 < 4; a0 += 2) {
-                    T old[2][4][4];
+    for (int a = 0; a < 2; ++a) {
+        const int ti = a == 0 ? W : 7 - W;
 #pragma unroll
-                    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 8; ++b)
+            if (t.on(a, b)) {
+                double *C = S + 256 * ((ti * (ti + 1)) / 2 + b) + l;
 #pragma unroll
-                        for (int b = 0; b < 4; ++b)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                int i = wr * 64 + t.row_of(a0 + a), j = wc * 64 + t.col_of(b, r);
-                                i = i < bs ? i : bs - 1;
-                                j = j < bs ? j : bs - 1;
-                                old[a][b][r] = Ck[i + (int64_t)j * lda];
-                            }
-#pragma unroll
-                    for (int a = 0; a < 2; ++a)
-#pragma unroll
-                        for (int b = 0; b < 4; ++b)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const int i = wr * 64 + t.row_of(a0 + a), j = wc * 64 + t.col_of(b, r);
-                                if (i >= j) {
-                                    const bool in = i < bs && j < bs;
-                                    S[sl(i, j)] = in ? (double)old[a][b][r] - (double)t.acc[a0 + a][b][r]
-                                                     : (i == j ? 1.0 : 0.0);
-                                }
-                            }
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * ti + (l & 15), j = 16 * b + 4 * r + (l >> 4);
+                    C[64 * r] = (i < bs && j < bs) ? t.acc[a][b][r] : (i == j ? 1.0 : 0.0);
                 }
             }
-        } else {
-            t.template store<false, false>(Ck, lda, wr * 64, wc * 64, rows, 128);
-        }
-    } else if (g == 0) {
-        block_to_lds(smem, Ck, lda, bs);
     }
+}
 
-    if (g == 0) {
-        const int f = potrf_in_lds(smem, Xl);  // begins with a barrier
-        if (f >= 0) {
-            if (tid == 0) atomicCAS(info, 0, (int)(k0 + f + 1));
-        } else {
-            lds_to_block(smem, Ck, lda, bs);
-        }
-        global_barrier();  // every wave's stores of L11 and the leaves have reached L2
-        if (tid == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        return;
+// The same for FP32 storage with low-precision MFMA (the mixed-precision factorization): 64 x
+// 64 wave tiles over the three lower quadrants, the old values read after the MFMAs, in two
+// halves of tile rows.
+template <typename T, int PREC>
+__device__ __forceinline__ void diag_update_lowp(double *S, const T *__restrict__ A, int64_t lda, int64_t k0, int bs)
+{
+    const int w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    if (wr == 0 && wc == 1) return;  // strictly upper quadrant
+    const T *Ck = A + k0 + k0 * lda;
+    using Tile = typename TileSel<T, PREC>::type;
+    Tile t;
+    t.zero();
+    t.setup(A + k0 + (k0 - 128) * lda, lda, wr * 64, bs, A + k0 + (k0 - 128) * lda, lda, wc * 64, bs, 128);
+    t.template run_fixed<128 / Tile::KS>();
+#pragma unroll
+    for (int a0 = 0; a0 < 4; a0 += 2) {
+        T old[2][4][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int i = wr * 64 + t.row_of(a0 + a), j = wc * 64 + t.col_of(b, r);
+                    i = i < bs ? i : bs - 1;
+                    j = j < bs ? j : bs - 1;
+                    old[a][b][r] = Ck[i + (int64_t)j * lda];
+                }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = wr * 64 + t.row_of(a0 + a), j = wc * 64 + t.col_of(b, r);
+                    if (i >= j) {
+                        const bool in = i < bs && j < bs;
+                        S[sl(i, j)] = in ? (double)old[a][b][r] - (double)t.acc[a0 + a][b][r] : (i == j ? 1.0 : 0.0);
+                    }
+                }
     }
-    global_barrier();  // this role's updated rows are in L2 before other waves read them back
-    if (tid == 0) {
+}
+
+// The diagonal block's strictly lower tiles in S (sl order) -> the panel image W (sls order).
+__device__ __forceinline__ void publish_lower_tiles(const double *S, double *__restrict__ W)
+{
+#pragma unroll
+    for (int q = 0; q < 28; ++q) {
+        const int ti = 1 + (q >= 1) + (q >= 3) + (q >= 6) + (q >= 10) + (q >= 15) + (q >= 21);
+        W[256 * q + threadIdx.x] = S[256 * (q + ti) + threadIdx.x];  // sl tile index = sls index + ti
+    }
+}
+
+// Waits (thread 0) until *flag >= epoch, then every wave acquires: each invalidates its own
+// view of the caches, so that the loads it issues next see what the flag's release published.
+// Returns false (uniformly) on a timeout, recorded as MI_PANEL_TIMEOUT.
+__device__ __forceinline__ bool wait_panel_flag(int *s_ok, const uint32_t *flag, uint32_t epoch, int *status)
+{
+    if (threadIdx.x == 0) {
         uint32_t spins = 0;
         int ok = 1;
         while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
@@ -363,12 +413,107 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__
                 break;
             }
         }
-        if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *s_ok = ok;
     }
     __syncthreads();
-    if (!*s_ok || factor_failed(info)) return;
-    trsm_rows<false, T>(smem, Ck, lda, rows, A + k0 + k0 * lda, lda, 128, Xl, nullptr);
+    const bool ok = *s_ok != 0;
+    if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return ok;
+}
+
+// The panel role of one block step; g is the role from the ticket:
+//   role 0   : C = A_kk - P_0 P_0^T straight into LDS, Cholesky in LDS, the panel image
+//              (L11's strictly lower tiles and the leaf inverses) to the workspace, publish
+//              `flag` = epoch (release), then L11 into A;
+//   role g>0 : rows k0+128g..+127: C = A_gk - P_g P_0^T, wait for the flag, then rows <-
+//              rows * L11^{-T} with the published image, stored once.
+// The diagonal role publishes even on a failing pivot, so that no waiter is left behind;
+// waiters then see `info` and stop.  (LLT.h:350-366: the update of block column k by all
+// earlier panels precedes its unblocked factor and the right-side solve.)
+// smem: kSmem doubles of LDS; s_ok: an LDS int.  T/PREC: storage and update arithmetic
+// (the diagonal factor and the panel solve are FP64 for every T).  With T = double the row
+// role keeps its rows in registers from the update to the store; with T = float the update
+// is rounded to FP32 storage first (as the mixed-precision trailing updates are) and read back.
+template <typename T, int PREC>
+__device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__restrict__ A,
+                                           int64_t lda, int64_t n, int64_t k0, double *__restrict__ Wleaf, int *info,
+                                           uint32_t *flag, uint32_t epoch, int *status)
+{
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t R0 = k0 + 128 * (int64_t)g;
+    const int rows = (int)((n - R0) < 128 ? (n - R0) : 128);
+    const int bs = (int)((n - k0) < 128 ? (n - k0) : 128);
+    T *Ck = A + R0 + k0 * lda;  // this role's 128 rows of block column k0
+    double *Wp = Wleaf + (size_t)(k0 / 128) * kPanelDoubles;  // the panel image
+    double *Xl = Wp + kLeafOffset;
+
+    if (g == 0) {
+        if (k0 > 0) {
+            if constexpr (sizeof(T) == 8) {
+                const double *P0 = A + k0 + (k0 - 128) * lda;
+                switch (w) {
+                case 0: diag_update_f64<0>(smem, Ck, P0, lda, bs); break;
+                case 1: diag_update_f64<1>(smem, Ck, P0, lda, bs); break;
+                case 2: diag_update_f64<2>(smem, Ck, P0, lda, bs); break;
+                default: diag_update_f64<3>(smem, Ck, P0, lda, bs); break;
+                }
+            } else {
+                diag_update_lowp<T, PREC>(smem, A, lda, k0, bs);
+            }
+        } else {
+            block_to_lds(smem, Ck, lda, bs);
+        }
+        const int f = potrf_in_lds(smem, Xl);  // begins with a barrier
+        if (f >= 0) {
+            if (tid == 0) atomicCAS(info, 0, (int)(k0 + f + 1));
+        } else {
+            publish_lower_tiles(smem, Wp);
+        }
+        global_barrier();  // every wave's stores of the image have reached L2
+        if (tid == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (f < 0) lds_to_block(smem, Ck, lda, bs);  // read by later launches only
+        return;
+    }
+
+    if constexpr (sizeof(T) == 8) {
+        WaveTile<2, 8, 2, true, 0xffffffffu, true> t;  // P_0 has all 128 rows (k0 + 128 < n)
+        const int r2[2] = {32 * w, 32 * w + 16};
+        t.zero();
+        t.load_acc(Ck, lda, r2, 0, rows, 128);
+        if (k0 > 0) {  // C -= P_g P_0^T, P = previous panel (columns k0-128..k0-1): rows R0.. and k0..
+            t.setup_rows(A + R0 + (k0 - 128) * lda, lda, r2, rows, A + k0 + (k0 - 128) * lda, lda, 0, 128, 128);
+            t.template run_fixed<32>();
+        }
+        if (!wait_panel_flag(s_ok, flag, epoch, status) || factor_failed(info)) return;
+        stage_panel_image(smem, Wp);
+        __syncthreads();
+        trsm_in_registers(smem, t.acc);
+        const int l = lane_id();
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int i = r2[a] + (l & 15);
+            if (i < rows) {
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Ck[i + (int64_t)(16 * b + 4 * r + (l >> 4)) * lda] = t.acc[a][b][r];
+            }
+        }
+    } else {
+        if (k0 > 0) {
+            const int wr = w >> 1, wc = w & 1;
+            using Tile = typename TileSel<T, PREC>::type;
+            Tile t;
+            t.zero();
+            t.setup(A + R0 + (k0 - 128) * lda, lda, wr * 64, rows, A + k0 + (k0 - 128) * lda, lda, wc * 64, bs, 128);
+            t.template run_fixed<128 / Tile::KS>();
+            t.template store<false, false>(Ck, lda, wr * 64, wc * 64, rows, 128);
+        }
+        global_barrier();  // this role's updated rows are in L2 before other waves read them back
+        if (!wait_panel_flag(s_ok, flag, epoch, status) || factor_failed(info)) return;
+        trsm_rows<false, T>(smem, Ck, lda, rows, (const T *)nullptr, 0, 128, nullptr, nullptr, Wp);
+    }
 }
 
 }  // namespace mi

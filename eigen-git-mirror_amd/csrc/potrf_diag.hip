@@ -30,8 +30,8 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict_
     const int b = (int)blockIdx.x;
     const int64_t kb = (int64_t)b * 128;
     const int bs = (int)((n - kb) < 128 ? (n - kb) : 128);
-    trsm_rows<true, T>(smem, (T *)nullptr, 0, 128, L + kb + kb * ldl, ldl, bs, Wleaf + (size_t)b * kLeafDoubles,
-                       Wout + (size_t)b * kInvDoubles);
+    trsm_rows<true, T>(smem, (T *)nullptr, 0, 128, L + kb + kb * ldl, ldl, bs,
+                       Wleaf + (size_t)b * kPanelDoubles + kLeafOffset, Wout + (size_t)b * kInvDoubles);
 }
 
 // Leaf inverses of the diagonal blocks of an already factored matrix: workgroup b, wave w
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void leaf_inverse_kernel(const T *__restrict__
         if (l < 16) {
 #pragma unroll
             for (int c = 0; c < 16; ++c)
-                Wleaf[(size_t)blockIdx.x * 8 * 256 + p * 256 + l + 16 * c] = (c <= l) ? x[c] : 0.0;
+                Wleaf[(size_t)blockIdx.x * kPanelDoubles + kLeafOffset + p * 256 + l + 16 * c] = (c <= l) ? x[c] : 0.0;
         }
     }
 }
