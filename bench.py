@@ -328,9 +328,16 @@ def main():
     # update by panel k), FP64 MFMA; achieved = algorithmic flops of the launches / their time
     st_ms, st_fl, st_n = tr["factor_step"]
     achieved = st_fl / (st_ms * 1e-3) / 1e12
+    # HBM bytes per launch: rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a
+    # copy kernel of known size (committed summary); none for other sizes
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "factor_step_fp64_n8192_traffic.json")
+    if n == 8192 and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["hbm_bytes_per_dispatch"]
     roofline = {"bound": "mfma", "kernel": "factor_step_kernel (panel POTRF/TRSM + trailing SYRK, FP64 MFMA)",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "per_launch": {"avg_ms": st_ms / st_n, "avg_algorithmic_flops": st_fl / st_n,
                                "launches": st_n}}
 
