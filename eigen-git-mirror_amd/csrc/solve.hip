@@ -12,8 +12,9 @@
 // item from an atomic ticket counter, in dependency order, so every block it waits for
 // belongs to a workgroup that started earlier and is resident: the waits always end.  Each
 // finished block is published with a release store of the call's epoch into its flag;
-// consumers poll with relaxed loads and issue one acquire fence once the flag is set.  Flags carry an increasing epoch, so they never need
-// clearing; only the ticket counter is reset (by a one-thread kernel) before each sweep.
+// consumers poll with relaxed loads and issue one acquire fence once the flag is set.  Flags
+// carry an increasing epoch, so they never need clearing; only the ticket counter is reset
+// (by a one-thread kernel) before each sweep.
 // A poll that exceeds a large bound records MI_SOLVE_TIMEOUT in the status word and the
 // item is abandoned instead of spinning forever.
 //
@@ -32,12 +33,6 @@ constexpr int SB = 128;          // row block (= factorization block)
 constexpr int SC = 16;           // right-hand sides per work item
 constexpr uint32_t kSpinLimit = 1u << 22;  // ~0.2 s of polling
 
-// Waits until *flag >= epoch (called by one thread; the caller's barrier then orders the
-// workgroup's reads after it).  Polls are relaxed agent-scope loads, which read the
-// coherent value without invalidating caches; a single acquire fence follows success.  (An
-// acquire on every poll invalidates the XCD's L2 each time, and with hundreds of polling
-// workgroups that starves every other memory access.)  Gives up, recording
-// MI_SOLVE_TIMEOUT, after kSpinLimit polls, or at once when another workgroup gave up.
 // Workgroup barrier ordering LDS accesses only (outstanding global loads stay in flight).
 __device__ __forceinline__ void lds_barrier()
 {
@@ -46,6 +41,12 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Waits until *flag >= epoch (called by one thread; the caller's barrier then orders the
+// workgroup's reads after it).  Polls are relaxed agent-scope loads, which read the
+// coherent value without invalidating caches; a single acquire fence follows success.  (An
+// acquire on every poll invalidates the caches each time, and with hundreds of polling
+// workgroups that starves every other memory access.)  Gives up, recording
+// MI_SOLVE_TIMEOUT, after kSpinLimit polls, or at once when another workgroup gave up.
 __device__ __forceinline__ bool wait_flag(const uint32_t *flag, uint32_t epoch, int *status)
 {
     uint32_t spins = 0;
@@ -65,6 +66,10 @@ __device__ __forceinline__ bool wait_flag(const uint32_t *flag, uint32_t epoch, 
 // BACKWARD = false: forward sweep with L; true: backward sweep with L^T.
 // Winv holds, per block b, X_b^T (ld 128) at Winv + b*2*16384 and X_b at +16384.
 // T: storage of L (float for a mixed-precision factor, widened on load; FP64 arithmetic).
+//
+// Each wave owns one 16-row tile of the block, rt = w for waves 0..3 and 11 - w for waves
+// 4..7: waves w and w+4 share a SIMD, and the triangular X_i product costs row tile rt
+// 4(rt+1) MFMA steps forward (4(8-rt) backward), so every SIMD issues 36 of them.
 template <bool BACKWARD, typename T>
 __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ L, int64_t lda, int64_t n,
                                                           double *__restrict__ B, int64_t ldb, int64_t nrhs,
@@ -78,11 +83,12 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
     __shared__ int s_ok;
     const int nblk = (int)((n + SB - 1) / SB);
     const int ngrp = (int)((nrhs + SC - 1) / SC);
-    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    const int tid = threadIdx.x, l = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     if (tid == 0) s_item = atomicAdd(ticket, 1);
     __syncthreads();
-    const int t = s_item;
+    const int t = __builtin_amdgcn_readfirstlane(s_item);
     if (t >= nblk * ngrp) return;
     const int ib = BACKWARD ? nblk - 1 - t / ngrp : t / ngrp;
     const int g = t % ngrp;
@@ -92,38 +98,38 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
     uint32_t *gflags = flags + (size_t)g * nblk;
 
     // this wave's 16 rows of the block
-    const int r = 16 * w + (l & 15);
+    const int rt = w < 4 ? w : 11 - w;
+    const int r = 16 * rt + (l & 15);
     const int rc = r < bi ? r : bi - 1;  // clamped row (in bounds)
     const int kl = l >> 4;
 
+    // B_i (this wave's rows; lane l, element e: row r, column 4e + l/16): read before any wait
+    double bv[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int c = 4 * e + kl;
+        bv[e] = B[(i0 + rc) + (c0 + (c < nc ? c : 0)) * ldb];
+    }
+
     // P operand of a contributing block jb, all 32 k-steps of this lane at once:
-    // P[row][k] = L_ij[row][k] (forward) or L_ji[k][row] (backward); k clamped in bounds.  After the last contributing block the same registers receive
-    // op(X_i): column-major P[row][k] = X_i[row][k] (forward) or X_i[k][row] (backward).
-    const double *Xp = Winv + (size_t)ib * 2 * SB * SB + (BACKWARD ? 0 : SB * SB);
+    // P[row][k] = L_ij[row][k] (forward) or L_ji[k][row] (backward); k clamped in bounds.
     const int jstep = BACKWARD ? -1 : 1;
     double pv[SB / 4];
 #define MI_LOAD_OPERAND(jb_)                                                                       \
     do {                                                                                           \
-        if ((jb_) != ib) {                                                                         \
-            const int64_t j0_ = (int64_t)(jb_)*SB;                                                 \
-            const int bj_ = (int)((n - j0_) < SB ? (n - j0_) : SB);                                \
-            /* uniform part of the address per step (scalar), lane part once; k clamped into a \
-               partial block */                                                                \
-            const T *colbase_ = BACKWARD ? L + j0_ + (i0 + rc) * lda : L + (i0 + rc) + j0_ * lda;   \
-            _Pragma("unroll") for (int s = 0; s < SB / 4; ++s)                                     \
-            {                                                                                      \
-                const int k_ = (4 * s + kl) < bj_ ? (4 * s + kl) : bj_ - 1;                         \
-                pv[s] = (double)colbase_[BACKWARD ? (int64_t)k_ : (int64_t)k_ * lda];              \
-            }                                                                                      \
-        } else {                                                                                   \
-            const double *p_ = Xp + r + SB * kl;                                                   \
-            _Pragma("unroll") for (int s = 0; s < SB / 4; ++s) { pv[s] = *p_; p_ += 4 * SB; }      \
+        const int64_t j0_ = (int64_t)(jb_)*SB;                                                     \
+        const int bj_ = (int)((n - j0_) < SB ? (n - j0_) : SB);                                    \
+        const T *colbase_ = BACKWARD ? L + j0_ + (i0 + rc) * lda : L + (i0 + rc) + j0_ * lda;       \
+        _Pragma("unroll") for (int s = 0; s < SB / 4; ++s)                                         \
+        {                                                                                          \
+            const int k_ = (4 * s + kl) < bj_ ? (4 * s + kl) : bj_ - 1;                             \
+            pv[s] = (double)colbase_[BACKWARD ? (int64_t)k_ : (int64_t)k_ * lda];                  \
         }                                                                                          \
     } while (0)
 
     dvec4 acc = {0.0, 0.0, 0.0, 0.0};
     int jb = BACKWARD ? nblk - 1 : 0;
-    MI_LOAD_OPERAND(jb);
+    if (jb != ib) MI_LOAD_OPERAND(jb);
     for (; jb != ib; jb += jstep) {
         if (tid == 0) s_ok = wait_flag(gflags + jb, epoch, status) ? 1 : 0;
         __syncthreads();
@@ -141,24 +147,33 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         // clamped (finite) P values past a partial block contribute nothing
 #pragma unroll
         for (int s = 0; s < SB / 4; ++s) acc = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], acc);
-        MI_LOAD_OPERAND(jb + jstep);  // next block (or op(X_i)): in flight during the next wait
-        lds_barrier();                // Ys is rewritten by the next contribution
+        if (jb + jstep != ib) MI_LOAD_OPERAND(jb + jstep);  // next block: in flight during the next wait
+        lds_barrier();  // Ys is rewritten by the next contribution
     }
 #undef MI_LOAD_OPERAND
+
+    // op(X_i): column-major P[row][k] = X_i[row][k] (forward, lower: k < 16(rt+1)) or X_i[k][row]
+    // (backward, upper: k >= 16 rt); only the nonzero k-steps are loaded and multiplied
+    const double *Xp = Winv + (size_t)ib * 2 * SB * SB + (BACKWARD ? 0 : SB * SB) + r + SB * kl;
+    const int s_lo = BACKWARD ? 4 * rt : 0;
+    const int s_hi = BACKWARD ? SB / 4 : 4 * (rt + 1);
+#pragma unroll
+    for (int s = 0; s < SB / 4; ++s)
+        if (s >= s_lo && s < s_hi) pv[s] = Xp[(size_t)4 * SB * s];
 
     // T = B_i - acc  (lane l, element e: row r, column 4e + l/16)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int c = 4 * e + kl;
         const bool in = r < bi && c < nc;
-        const double bv = B[(i0 + rc) + (c0 + (c < nc ? c : 0)) * ldb];
-        Ts[c + SC * r] = in ? bv - acc[e] : 0.0;
+        Ts[c + SC * r] = in ? bv[e] - acc[e] : 0.0;
     }
     lds_barrier();
     // Y_i = op(X_i) T
     dvec4 y = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < SB / 4; ++s) y = mfma_f64(Ts[(l & 15) + SC * (4 * s + kl)], pv[s], y);
+    for (int s = 0; s < SB / 4; ++s)
+        if (s >= s_lo && s < s_hi) y = mfma_f64(Ts[(l & 15) + SC * (4 * s + kl)], pv[s], y);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int c = 4 * e + kl;
