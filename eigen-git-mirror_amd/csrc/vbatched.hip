@@ -47,6 +47,7 @@ struct VbArith<double> {
     using acc_t = dvec4;
     __device__ static acc_t mma(double q, double p, acc_t c) { return mfma_f64(q, p, c); }
     __device__ static int kx(int s, int g) { return 4 * s + g; }
+    __device__ static constexpr int step(int s) { return 4 * s; }  // kx(s, g) - kx(0, g)
     __device__ static double rsqrt(double x)
     {
         const double r = __builtin_amdgcn_rsq(x);
@@ -60,6 +61,7 @@ struct VbArith<float> {
     using acc_t = fvec4;
     __device__ static acc_t mma(float q, float p, acc_t c) { return mfma_f32(q, p, c); }
     __device__ static int kx(int s, int g) { return 4 * g + s; }
+    __device__ static constexpr int step(int s) { return s; }  // kx(s, g) - kx(0, g)
     __device__ static float rsqrt(float x)
     {
         const float r = __builtin_amdgcn_rsqf(x);
@@ -78,6 +80,12 @@ __device__ __forceinline__ void vb_fence()
 
 // Factor + solve one system with one wave.  Ls, Xs: 256 elements of LDS each (this wave's);
 // Xg: kVbMaxT * 256 elements of global scratch (this wave's).  Returns LAPACK info.
+//
+// Loads of a k-step s of column block j use a wave-uniform base (A + (16j + step(s)) * lda)
+// and a per-lane 32-bit byte offset of (row, lane group's column), the scalar-base form of
+// the global loads.  Every left-looking loop over j is software-pipelined: the loads of step
+// j+1 go out before the MFMAs of step j (two register buffers used alternately by name, so
+// no buffer is indexed at run time).
 template <typename T, int G>
 __device__ int vb_system(int n, T *__restrict__ A, int64_t lda, int nrhs, T *__restrict__ B, int64_t ldb, T *Ls,
                          T *Xs, T *__restrict__ Xg)
@@ -87,63 +95,84 @@ __device__ int vb_system(int n, T *__restrict__ A, int64_t lda, int nrhs, T *__r
     const int l = lane_id(), li = l & 15, g = l >> 4;
     const int nt = (n + 15) >> 4;
     const T zero = (T)0, one = (T)1;
+    const int kx0 = M::kx(0, g);  // kx(s, g) = kx0 + M::step(s)
+    auto at = [](const T *base, uint32_t off) -> T {
+        return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+    };
+    auto lane_off = [&](int row, int64_t ld) -> uint32_t { return (uint32_t)((row + (int64_t)kx0 * ld) * (int64_t)sizeof(T)); };
 
     for (int k = 0; k < nt; ++k) {
         const int k0 = 16 * k;
-        // ---- C_kk = A_kk - sum_{j<k} L_kj L_kj^T
-        acc_t acc = {zero, zero, zero, zero};
+        const int rk = k0 + li < n ? k0 + li : n - 1;
+        const uint32_t offq = lane_off(rk, lda);
+        // ---- C_kk = A_kk - sum_{j<k} L_kj L_kj^T -> Ls (identity beyond n)
         {
-            const int rk = k0 + li < n ? k0 + li : n - 1;
-            for (int j = 0; j < k; ++j) {
-                T q[4];
+            acc_t acc = {zero, zero, zero, zero};
+            const bool kin = k0 + li < n;
+            T qa[4], qb[4];
+#define VB_LOADD(q_, j_)                                                                           \
+    do {                                                                                           \
+        _Pragma("unroll") for (int s = 0; s < 4; ++s) q_[s] = at(A + (int64_t)(16 * (j_) + M::step(s)) * lda, offq); \
+    } while (0)
+#define VB_MMAD(q_)                                                                                \
+    do {                                                                                           \
+        _Pragma("unroll") for (int s = 0; s < 4; ++s) acc = M::mma(kin ? q_[s] : zero, kin ? q_[s] : zero, acc); \
+    } while (0)
+            if (k > 0) {
+                VB_LOADD(qa, 0);
+                for (int j = 0; j < k; j += 2) {
+                    if (j + 1 < k) VB_LOADD(qb, j + 1);
+                    VB_MMAD(qa);
+                    if (j + 1 >= k) break;
+                    if (j + 2 < k) VB_LOADD(qa, j + 2);
+                    VB_MMAD(qb);
+                }
+            }
+#undef VB_LOADD
+#undef VB_MMAD
 #pragma unroll
-                for (int s = 0; s < 4; ++s) q[s] = A[rk + (int64_t)(16 * j + M::kx(s, g)) * lda];
-                const bool ok = k0 + li < n;
-#pragma unroll
-                for (int s = 0; s < 4; ++s) acc = M::mma(ok ? q[s] : zero, ok ? q[s] : zero, acc);
+            for (int r = 0; r < 4; ++r) {
+                const int row = li, col = M::kx(r, g);
+                const int gi = k0 + row, gj = k0 + col;
+                const bool in = gi < n && gj < n;
+                const T a = A[(in ? gi : 0) + (int64_t)(in ? gj : 0) * lda];
+                Ls[row + 16 * col] = in ? (T)(a - acc[r]) : (row == col ? one : zero);
             }
         }
+        // ---- leaf: lanes 0..15 rows of C_kk, lanes 16..31 identity rows; the column operations
+        // run on every lane without lane predicates (entries above a leaf lane's diagonal change
+        // but are never broadcast nor stored), column j+1 -- the next pivot -- first
+        {
+            T v[16];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = li, col = M::kx(r, g);
-            const int gi = k0 + row, gj = k0 + col;
-            const bool in = gi < n && gj < n;
-            const T a = A[(in ? gi : 0) + (int64_t)(in ? gj : 0) * lda];
-            Ls[row + 16 * col] = in ? (T)(a - acc[r]) : (row == col ? one : zero);
-        }
-        // ---- leaf: lanes 0..15 rows of C_kk (lower), lanes 16..31 identity rows
-        T v[16];
+            for (int c = 0; c < 16; ++c) v[c] = l < 16 ? Ls[li + 16 * c] : ((l < 32 && c == l - 16) ? one : zero);
+            int fail = -1;
+            T x = vb_readlane(v[0], 0);
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            const T s = Ls[(li) + 16 * c];
-            v[c] = l < 16 ? (c <= l ? s : zero) : (l < 32 && c == l - 16 ? one : zero);
-        }
-        int fail = -1;
+            for (int j = 0; j < 16; ++j) {
+                if (fail < 0 && x <= zero) fail = j;  // LLT.h:327-330; a NaN pivot passes
+                v[j] *= M::rsqrt(x);
+                if (j + 1 < 16) {
+                    v[j + 1] -= v[j] * vb_readlane(v[j], j + 1);
+                    x = vb_readlane(v[j + 1], j + 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const T x = vb_readlane(v[j], j);
-            if (fail < 0 && x <= zero) fail = j;  // LLT.h:327-330; a NaN pivot passes
-            const T rd = M::rsqrt(x);
-            const T d = x * rd;
-            v[j] = l == j ? d : (l > j ? v[j] * rd : v[j]);
-#pragma unroll
-            for (int c = j + 1; c < 16; ++c) {
-                const T lc = vb_readlane(v[j], c);
-                v[c] -= v[j] * lc;
+                for (int c = j + 2; c < 16; ++c) v[c] -= v[j] * vb_readlane(v[j], c);
             }
-        }
-        if (fail >= 0) return k0 + fail + 1;
-        // L_kk to the system (lower part), X_kk = L_kk^{-1} to Xs / Xg (column-major)
-        if (l < 16 && k0 + l < n) {
+            if (fail >= 0) return k0 + fail + 1;
+            // L_kk to the system (lower part), X_kk = L_kk^{-1} to Xs / Xg (column-major)
+            if (l < 16 && k0 + l < n) {
 #pragma unroll
-            for (int c = 0; c < 16; ++c)
-                if (c <= l) A[(k0 + l) + (int64_t)(k0 + c) * lda] = v[c];
-        }
-        if (l >= 16 && l < 32) {
+                for (int c = 0; c < 16; ++c)
+                    if (c <= l) A[(k0 + l) + (int64_t)(k0 + c) * lda] = v[c];
+            }
+            if (l >= 16 && l < 32) {
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                Xs[c + 16 * (l - 16)] = v[c];  // lane 16+i holds row i of L^{-T} = column i of X
-                Xg[k * 256 + c + 16 * (l - 16)] = v[c];
+                for (int c = 0; c < 16; ++c) {
+                    Xs[c + 16 * (l - 16)] = v[c];  // lane 16+i holds row i of L^{-T} = column i of X
+                    Xg[k * 256 + c + 16 * (l - 16)] = v[c];
+                }
             }
         }
         // ---- panel: L_ik = (A_ik - sum_{j<k} L_ij L_kj^T) X_kk^T, i > k, G tile rows at a time
@@ -151,26 +180,46 @@ __device__ int vb_system(int n, T *__restrict__ A, int64_t lda, int nrhs, T *__r
 #pragma unroll
         for (int s = 0; s < 4; ++s) xq[s] = Xs[li + 16 * M::kx(s, g)];
         for (int i0 = k + 1; i0 < nt; i0 += G) {
+            uint32_t offp[G];
+            bool pin[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int row = 16 * (i0 + u) + li;
+                pin[u] = row < n;
+                offp[u] = lane_off(row < n ? row : n - 1, lda);
+            }
             acc_t pa[G];
 #pragma unroll
             for (int u = 0; u < G; ++u) pa[u] = acc_t{zero, zero, zero, zero};
-            for (int j = 0; j < k; ++j) {
-                T q[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) q[s] = A[(k0 + li) + (int64_t)(16 * j + M::kx(s, g)) * lda];
-#pragma unroll
-                for (int u = 0; u < G; ++u) {
-                    if (i0 + u < nt) {
-                        const int row = 16 * (i0 + u) + li;
-                        const int rc = row < n ? row : n - 1;
-                        T p[4];
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) p[s] = A[rc + (int64_t)(16 * j + M::kx(s, g)) * lda];
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) pa[u] = M::mma(q[s], row < n ? p[s] : zero, pa[u]);
-                    }
+            T qa[4], qb[4], xa[G][4], xb[G][4];
+#define VB_LOAD(q_, x_, j_)                                                                        \
+    do {                                                                                           \
+        _Pragma("unroll") for (int s = 0; s < 4; ++s)                                              \
+        {                                                                                          \
+            const T *base_ = A + (int64_t)(16 * (j_) + M::step(s)) * lda;                          \
+            q_[s] = at(base_, offq);                                                               \
+            _Pragma("unroll") for (int u = 0; u < G; ++u) x_[u][s] = at(base_, offp[u]);            \
+        }                                                                                          \
+    } while (0)
+#define VB_MMA(q_, x_)                                                                             \
+    do {                                                                                           \
+        _Pragma("unroll") for (int u = 0; u < G; ++u)                                              \
+        {                                                                                          \
+            _Pragma("unroll") for (int s = 0; s < 4; ++s) pa[u] = M::mma(q_[s], pin[u] ? x_[u][s] : zero, pa[u]); \
+        }                                                                                          \
+    } while (0)
+            if (k > 0) {
+                VB_LOAD(qa, xa, 0);
+                for (int j = 0; j < k; j += 2) {
+                    if (j + 1 < k) VB_LOAD(qb, xb, j + 1);
+                    VB_MMA(qa, xa);
+                    if (j + 1 >= k) break;
+                    if (j + 2 < k) VB_LOAD(qa, xa, j + 2);
+                    VB_MMA(qb, xb);
                 }
             }
+#undef VB_LOAD
+#undef VB_MMA
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 if (i0 + u < nt) {
@@ -201,21 +250,39 @@ __device__ int vb_system(int n, T *__restrict__ A, int64_t lda, int nrhs, T *__r
         const int col = c0 + li;
         const bool cin = col < nrhs;
         const int64_t bcol = (int64_t)(cin ? col : c0) * ldb;
+        const T *Bc = B + bcol;
         // forward: Y_k^T = (B_k^T - sum_{j<k} Y_j^T L_kj^T) X_kk^T
         for (int k = 0; k < nt; ++k) {
             const int k0 = 16 * k;
-            acc_t acc = {zero, zero, zero, zero};
             const int rk = k0 + li < n ? k0 + li : n - 1;
-            for (int j = 0; j < k; ++j) {
-                T p[4], q[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    p[s] = B[(16 * j + M::kx(s, g)) + bcol];
-                    q[s] = A[rk + (int64_t)(16 * j + M::kx(s, g)) * lda];
+            const uint32_t offq = lane_off(rk, lda);
+            const uint32_t offb = (uint32_t)(kx0 * (int64_t)sizeof(T));
+            acc_t acc = {zero, zero, zero, zero};
+            T pa_[4], qa_[4], pb_[4], qb_[4];
+#define VB_LOADF(p_, q_, j_)                                                                       \
+    do {                                                                                           \
+        _Pragma("unroll") for (int s = 0; s < 4; ++s)                                              \
+        {                                                                                          \
+            p_[s] = at(Bc + 16 * (j_) + M::step(s), offb);                                         \
+            q_[s] = at(A + (int64_t)(16 * (j_) + M::step(s)) * lda, offq);                         \
+        }                                                                                          \
+    } while (0)
+#define VB_MMAF(p_, q_)                                                                            \
+    do {                                                                                           \
+        _Pragma("unroll") for (int s = 0; s < 4; ++s) acc = M::mma(k0 + li < n ? q_[s] : zero, cin ? p_[s] : zero, acc); \
+    } while (0)
+            if (k > 0) {
+                VB_LOADF(pa_, qa_, 0);
+                for (int j = 0; j < k; j += 2) {
+                    if (j + 1 < k) VB_LOADF(pb_, qb_, j + 1);
+                    VB_MMAF(pa_, qa_);
+                    if (j + 1 >= k) break;
+                    if (j + 2 < k) VB_LOADF(pa_, qa_, j + 2);
+                    VB_MMAF(pb_, qb_);
                 }
-#pragma unroll
-                for (int s = 0; s < 4; ++s) acc = M::mma(k0 + li < n ? q[s] : zero, cin ? p[s] : zero, acc);
             }
+#undef VB_LOADF
+#undef VB_MMAF
             T c[4], xq2[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -237,21 +304,38 @@ __device__ int vb_system(int n, T *__restrict__ A, int64_t lda, int nrhs, T *__r
         // backward: X_k^T = (Y_k^T - sum_{j>k} X_j^T L_jk) X_kk
         for (int k = nt - 1; k >= 0; --k) {
             const int k0 = 16 * k;
+            // column k0 + li of L: below n whenever the loop runs (k < nt - 1); clamped anyway
+            const int64_t colkc = (int64_t)(k0 + li < n ? k0 + li : n - 1) * lda;
             acc_t acc = {zero, zero, zero, zero};
-            for (int j = k + 1; j < nt; ++j) {
-                T p[4], q[4];
-                bool ok[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int m = 16 * j + M::kx(s, g);
-                    ok[s] = m < n;
-                    const int mc = ok[s] ? m : n - 1;
-                    p[s] = B[mc + bcol];
-                    q[s] = A[mc + (int64_t)(k0 + li) * lda];
+            T pa_[4], qa_[4], pb_[4], qb_[4];
+            bool oka[4], okb[4];
+#define VB_LOADB(p_, q_, ok_, j_)                                                                  \
+    do {                                                                                           \
+        _Pragma("unroll") for (int s = 0; s < 4; ++s)                                              \
+        {                                                                                          \
+            const int m = 16 * (j_) + M::kx(s, g);                                                 \
+            ok_[s] = m < n;                                                                        \
+            const int mc = ok_[s] ? m : n - 1;                                                     \
+            p_[s] = B[mc + bcol];                                                                  \
+            q_[s] = A[mc + colkc];                                                                 \
+        }                                                                                          \
+    } while (0)
+#define VB_MMAB(p_, q_, ok_)                                                                       \
+    do {                                                                                           \
+        _Pragma("unroll") for (int s = 0; s < 4; ++s) acc = M::mma(ok_[s] ? q_[s] : zero, (ok_[s] && cin) ? p_[s] : zero, acc); \
+    } while (0)
+            if (k + 1 < nt) {
+                VB_LOADB(pa_, qa_, oka, k + 1);
+                for (int j = k + 1; j < nt; j += 2) {
+                    if (j + 1 < nt) VB_LOADB(pb_, qb_, okb, j + 1);
+                    VB_MMAB(pa_, qa_, oka);
+                    if (j + 1 >= nt) break;
+                    if (j + 2 < nt) VB_LOADB(pa_, qa_, oka, j + 2);
+                    VB_MMAB(pb_, qb_, okb);
                 }
-#pragma unroll
-                for (int s = 0; s < 4; ++s) acc = M::mma(ok[s] ? q[s] : zero, (ok[s] && cin) ? p[s] : zero, acc);
             }
+#undef VB_LOADB
+#undef VB_MMAB
             T c[4], xq2[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -320,7 +404,7 @@ __global__ __launch_bounds__(1024) void vb_plan_kernel(int64_t batch, const int 
 }
 
 template <typename T, int G>
-__global__ __launch_bounds__(64 * kVbWaves) void vb_persistent_kernel(int64_t batch, const int *__restrict__ d_n,
+__global__ __launch_bounds__(64 * kVbWaves, 2) void vb_persistent_kernel(int64_t batch, const int *__restrict__ d_n,
                                                                       T *const *__restrict__ d_A,
                                                                       const int *__restrict__ d_lda,
                                                                       const int *__restrict__ d_nrhs,
