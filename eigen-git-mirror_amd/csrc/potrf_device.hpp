@@ -34,14 +34,16 @@ __device__ __forceinline__ double readlane_d(double x, int lane)
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Panel p (columns c0..c0+15 of the 128 x 128 block in S, tiled: sl), all four waves at once.
-// Every wave holds the 16 x 16 leaf in lanes 0..15 (row c0+l) and runs the column loop of
+// Panel p (columns c0..c0+15 of the 128 x 128 block in S, tiled: sl), waves 0..2 (wave 3 is
+// meanwhile busy with the previous panel's deferred update, see potrf_in_lds).  Each of the
+// three waves holds the 16 x 16 leaf in lanes 0..15 (row c0+l) and runs the column loop of
 // LLT.h:320-334 on it (pivot test, sqrt, scale, rank-1 update) redundantly, so its lane
-// broadcasts (readlane) need no barrier.  Lanes 16..63 of waves 0..2 hold the 48 rows
-// c0+16+48w+(l-16) below the leaf and receive the same column operations, which is the panel
-// solve A_ip L_pp^{-T}; lanes 16..31 of wave 3 hold the rows of the 16 x 16 identity, which the
-// same operations turn into L_pp^{-T}, i.e. column i of X_pp = L_pp^{-1} in lane 16+i, written
-// to xleaf column-major.  1/sqrt from the hardware estimate and one Newton step, d = x/sqrt(x).
+// broadcasts (readlane) need no barrier.  Lanes 16..63 hold the rows c0+16+48w+(l-16) below
+// the leaf (up to 112 rows over the three waves) and receive the same column operations, which
+// is the panel solve A_ip L_pp^{-T}; lanes 48..63 of wave 2 (never a row below) hold the rows
+// of the 16 x 16 identity, which the same operations turn into L_pp^{-T}, i.e. column i of
+// X_pp = L_pp^{-1} in lane 48+i, written to xleaf column-major.  1/sqrt from the hardware
+// estimate and one Newton step, d = x/sqrt(x).
 //
 // The column operations run on every lane without lane predicates: at step j, v[j] *= 1/sqrt(x)
 // gives sqrt(x) on lane j (where v[j] is the pivot x) and the scaled column below it, and the
@@ -50,23 +52,25 @@ __device__ __forceinline__ double readlane_d(double x, int lane)
 // are never broadcast (readlane reads lane c of column j < c) nor stored.  Without per-lane
 // selects the loop holds no predicate masks, so the broadcasts get scalar registers of their
 // own and column j+1 -- the next pivot -- is updated first, off the remaining updates.
+// (Broadcasting the multipliers through LDS instead -- the leaf lanes store column j, every
+// lane reads it back with uniform addresses -- measured slower: 23 vs 21.5 us per block.)
 //
-// Returns the failing panel column or -1, the same in every wave.  The factored leaf is left
-// in v (lanes 0..15): the caller writes it back to S only after a barrier, since the other
-// waves read the unfactored leaf from S here and may still be doing so when this wave is done.
-__device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restrict__ xleaf, double (&v)[16])
+// Returns the failing panel column or -1, the same in the three waves.  The factored leaf is
+// left in v (lanes 0..15): it is written back to S only after a barrier, since the other waves
+// read the unfactored leaf from S here and may still be doing so when one wave is done.
+__device__ __forceinline__ int panel_factor3(double *S, int c0, double *__restrict__ xleaf, double (&v)[16])
 {
     const int l = lane_id(), w = threadIdx.x >> 6;
     const bool leaf = l < 16;
     const int rb = c0 + 16 + 48 * w + (l - 16);
-    const bool below = !leaf && w < 3 && rb < 128;
-    const bool ident = !leaf && w == 3 && l < 32;
+    const bool ident = w == 2 && l >= 48;
+    const bool below = !leaf && !ident && rb < 128;
     const int r = leaf ? c0 + l : (below ? rb : c0);
 #pragma unroll
     for (int c = 0; c < 16; ++c) v[c] = S[sl(r, c0 + c)];
     if (ident) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) v[c] = (c == l - 16) ? 1.0 : 0.0;
+        for (int c = 0; c < 16; ++c) v[c] = (c == l - 48) ? 1.0 : 0.0;
     }
     int fail = -1;
     double x = readlane_d(v[0], 0);  // pivot: leaf row j lives in lane j
@@ -91,7 +95,7 @@ __device__ __forceinline__ int panel_factor4(double *S, int c0, double *__restri
     }
     if (ident) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) xleaf[c + 16 * (l - 16)] = v[c];
+        for (int c = 0; c < 16; ++c) xleaf[c + 16 * (l - 48)] = v[c];
     }
     return -1;
 }
@@ -135,31 +139,56 @@ __device__ __forceinline__ void lds_to_block(const double *S, T *__restrict__ Ab
     }
 }
 
-// Cholesky of the 128 x 128 lower block in S (all 256 threads), in eight 16-column panels:
-// panel factor and solve by all four waves (panel_factor4), then the rank-16 update of the
-// T = 7-p tile rows below by the panel on FP64 MFMA from LDS.  Every wave first loads the
-// panel's T tile-row operands (lane l, k-step s: row l%16, column 4s + l/16 -- the same layout
-// for both MFMA operands) into registers; the T(T+1)/2 lower tiles are then dealt round-robin
-// to the waves by a wave-uniform test over fully unrolled tile loops.  A diagonal tile is
-// updated whole: the entries above its diagonal are never read as data (the column loop keeps
-// them off its broadcasts and the copy out stores r >= c only).  Returns the first failing
-// column or -1 (uniform); writes the 8 leaf inverses to Xl.
-__device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
+// One 16 x 16 tile of the rank-16 update in LDS: C -= Pi Pj^T (operands in registers, lane l
+// k-step s: row l%16, column 4s + l/16), C at tile offset + l + 64 r.  A diagonal tile is
+// updated whole: the entries above its diagonal are never read as data.
+__device__ __forceinline__ void lds_tile_update(double *C, const double (&Pi)[4], const double (&Pj)[4])
 {
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = lane_id();
-    __syncthreads();
-    for (int p = 0; p < 8; ++p) {
-        const int c0 = 16 * p;
-        double lv[16];
-        const int f = panel_factor4(S, c0, Xl + p * 256, lv);
-        if (f >= 0) return c0 + f;
-        __syncthreads();  // every wave has read the leaf; the rows below it are in S
-        if (w == 0 && l < 16) {  // the leaf itself: read by nobody until the final copy out
+    double old[4];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) S[sl(c0 + l, c0 + c)] = lv[c];
-        }
+    for (int r = 0; r < 4; ++r) old[r] = C[64 * r];
+    dvec4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = mfma_f64(Pj[s], Pi[s], acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) C[64 * r] = old[r] - acc[r];
+}
+
+// Cholesky of the 128 x 128 lower block in S (all 256 threads), in eight 16-column panels with
+// a one-panel lookahead inside the block.  After panel p is factored, with T = 7-p tile rows
+// below it and their operands P_x (tile (p+1+x, p), x < T) in every wave's registers:
+//   A(p): the leaf of p goes back to S; the T tiles of column p+1 get panel p's update (the
+//         waves round-robin), barrier;
+//   B(p): waves 0..2 factor panel p+1 (panel_factor3: VALU and lane broadcasts) while wave 3
+//         applies panel p to the tiles right of column p+1 on the matrix core -- up to kCap of
+//         them, any more go to waves 0..2 after their column loop -- barrier.
+// Column p+1 thus has every earlier panel's update before its factor, the tiles beyond it get
+// panel p's before phase A(p+1) touches them, and the rank-16 updates mostly run under the
+// column loop.  Returns the first failing column or -1 (uniform); writes the 8 leaf inverses
+// to Xl.  s_fail: an LDS int.
+__device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl, int *s_fail)
+{
+    constexpr int kCap = 12;  // deferred tiles taken by wave 3 (about one column loop's worth)
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = lane_id();
+    double lv[16];
+    __syncthreads();
+    if (w < 3) {
+        const int f = panel_factor3(S, 0, Xl, lv);
+        if (w == 0 && l == 0) *s_fail = f;
+    }
+    __syncthreads();
+    {
+        const int f = __builtin_amdgcn_readfirstlane(*s_fail);
+        if (f >= 0) return f;
+    }
+    for (int p = 0; p < 7; ++p) {
         const int T = 7 - p;
-        const int tb = p + 1;  // first trailing tile row
+        const int tb = p + 1;  // first trailing tile row / column
+        // ---- A(p)
+        if (w == 0 && l < 16) {  // the leaf of p: read by nobody until the final copy out
+#pragma unroll
+            for (int c = 0; c < 16; ++c) S[sl(16 * p + l, 16 * p + c)] = lv[c];
+        }
         double P[7][4];
 #pragma unroll
         for (int x = 0; x < 7; ++x) {
@@ -170,25 +199,33 @@ __device__ __forceinline__ int potrf_in_lds(double *S, double *__restrict__ Xl)
             }
         }
 #pragma unroll
-        for (int ti = 0; ti < 7; ++ti) {
+        for (int x = 0; x < 7; ++x)
+            if (x < T && (x & 3) == w) lds_tile_update(S + 256 * (((tb + x) * (tb + x + 1)) / 2 + tb) + l, P[x], P[0]);
+        __syncthreads();
+        // ---- B(p)
+        if (w < 3) {
+            const int f = panel_factor3(S, 16 * tb, Xl + 256 * tb, lv);
+            if (w == 0 && l == 0) *s_fail = f;
+        }
 #pragma unroll
-            for (int tj = 0; tj <= ti; ++tj) {
-                const int t = (ti * (ti + 1)) / 2 + tj;
-                if (ti < T && (t & 3) == w) {
-                    double *C = S + 256 * (((tb + ti) * (tb + ti + 1)) / 2 + tb + tj) + l;
-                    double old[4];
+        for (int ti = 1; ti < 7; ++ti) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) old[r] = C[64 * r];
-                    dvec4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) acc = mfma_f64(P[tj][s], P[ti][s], acc);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) C[64 * r] = old[r] - acc[r];
-                }
+            for (int tj = 1; tj <= ti; ++tj) {
+                const int t = ((ti - 1) * ti) / 2 + (tj - 1);  // enumerates the deferred tiles
+                const int owner = t < kCap ? 3 : (t - kCap) % 3;
+                if (ti < T && w == owner)
+                    lds_tile_update(S + 256 * (((tb + ti) * (tb + ti + 1)) / 2 + tb + tj) + l, P[ti], P[tj]);
             }
         }
         __syncthreads();
+        const int f = __builtin_amdgcn_readfirstlane(*s_fail);
+        if (f >= 0) return 16 * tb + f;
     }
+    if (w == 0 && l < 16) {  // the last leaf
+#pragma unroll
+        for (int c = 0; c < 16; ++c) S[sl(112 + l, 112 + c)] = lv[c];
+    }
+    __syncthreads();
     return -1;
 }
 
@@ -464,7 +501,7 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__
         } else {
             block_to_lds(smem, Ck, lda, bs);
         }
-        const int f = potrf_in_lds(smem, Xl);  // begins with a barrier
+        const int f = potrf_in_lds(smem, Xl, s_ok);  // begins with a barrier
         if (f >= 0) {
             if (tid == 0) atomicCAS(info, 0, (int)(k0 + f + 1));
         } else {
