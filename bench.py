@@ -225,11 +225,16 @@ def leg_adjoint(n=4096, nb=512, batch=256, reps=3):
 
     t_f, _ = _time_ms(fwd, reps)
     # adjoint flops: tril(L^T Lbar) n^3/3 + two triangular solves with n right-hand sides 2 n^3
+    # (the textbook count); the library's second solve computes only the block-upper half of
+    # the symmetric result, n^3/3 + n^3 + n^3/2 performed
     adj_fl = n ** 3 / 3 + 2.0 * n ** 3
+    adj_done = n ** 3 / 3 + 1.5 * n ** 3
     t_adj = max(t_all - t_f, 1e-6)
     out["single"] = {"n": n, "forward_ms": round(t_f, 3), "forward_backward_ms": round(t_all, 3),
                      "adjoint_ms": round(t_adj, 3), "adjoint_tflops": round(adj_fl / t_adj / 1e9, 3),
-                     "adjoint_frac_fp64_peak": round(adj_fl / t_adj / 1e9 / PEAK_FP64_TFLOPS, 4)}
+                     "adjoint_frac_fp64_peak": round(adj_fl / t_adj / 1e9 / PEAK_FP64_TFLOPS, 4),
+                     "adjoint_tflops_performed": round(adj_done / t_adj / 1e9, 3),
+                     "adjoint_frac_fp64_peak_performed": round(adj_done / t_adj / 1e9 / PEAK_FP64_TFLOPS, 4)}
     ks = [workloads.gp_kernel_torch(nb, seed=100 + i, device=dev) for i in range(batch)]
 
     def batch_all():

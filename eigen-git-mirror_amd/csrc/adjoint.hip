@@ -6,9 +6,11 @@
 // decomposition", 2016, with the symmetric convention of torch.linalg.cholesky):
 //   P    = tril(L^T Lbar)                                  (triangular-times-triangular)
 //   M    = (tril(P) + tril(P, -1)^T) / 2                   (symmetric completion, halved)
-//   Abar = sym(L^{-T} M L^{-1}):  Z = L^{-T} M,  S = L^{-T} Z^T,  Abar = (S + S^T) / 2
-// The two triangular solves reuse the persistent backward sweep (n right-hand sides) with
-// the diagonal-block inverses recomputed from L (nothing of the forward pass is stored).
+//   Abar = sym(L^{-T} M L^{-1}):  Y = M L^{-1} (= (L^{-T} M)^T),  S = Y^T L^{-1},  Abar = sym(S)
+// The two triangular solves run as blocked right-side substitutions over n rows
+// (trsm_multi.hip, operands row-contiguous through a transposed copy of L) with the
+// diagonal-block inverses recomputed from L (nothing of the forward pass is stored); the
+// second computes only the block-upper half of the symmetric S.
 // For X = A^{-1} B (solve with the factor) and Xbar: Bbar = A^{-1} Xbar and
 //   Lbar -= tril(Bbar (L^T X)^T + X (L^T Bbar)^T)          (O(n^2 r), no n^3 product).
 #include "common.hpp"
@@ -17,85 +19,89 @@
 
 namespace mi {
 
-// P[i][j] = sum_{k >= i} L[k][i] G[k][j] for i >= j (tril(L^T G), L and G lower; their
-// upper triangles are never read).  128 x 128 tiles of P's lower triangle, 4 waves of 64 x 64
-// (4 x 4 FP64 MFMA tiles); operands k-major: element (row r, k) at X[k + r * ld].
-__global__ __launch_bounds__(256) void trmm_tn_lower_kernel(const double *__restrict__ L, int64_t ldl,
-                                                            const double *__restrict__ G, int64_t ldg, int64_t n,
-                                                            double *__restrict__ P, int64_t ldp)
+// P[i][j] = sum_{k >= i} L[k][i] G[k][j] for i >= j (tril(L^T G), L and G lower), from the
+// transposed lower parts Lt = tril(L)^T and Gt = tril(G)^T (zero above their diagonals), so that
+// P = Lt Gt^T is a plain product with row-contiguous operands.  128 x 128 tiles of P's lower
+// triangle, 4 waves of 64 x 64 (RowTile64).
+//
+// Tile row ti needs k in [128 ti, n): the first tiles' depth is n, the last ones' 128.  So each
+// tile's k range is split into chunks of ck (a multiple of 128, about n/3), one workgroup per
+// (tile, chunk) writing its 128 x 128 partial (ld 128) to Wp[item]; trmm_reduce_kernel then
+// adds each tile's partials in chunk order (deterministic).  The launcher grows ck until the
+// partials fit in the n^2 doubles of scratch; when even one chunk per tile would not (n below
+// about 300), each tile is written straight to its place in an n x n output (its entries above
+// the diagonal too: nothing reads them).  Items are numbered tile row by tile row, (tj, chunk)
+// within a row.
+__device__ __forceinline__ int trmm_chunks(int64_t n, int ti, int64_t ck)
 {
-    const int T = (int)((n + 127) / 128);
+    return (int)((n - 128 * (int64_t)ti + ck - 1) / ck);
+}
+
+__global__ __launch_bounds__(256, 2) void trmm_part_kernel(const double *__restrict__ Lt, int64_t ldlt,
+                                                           const double *__restrict__ Gt, int64_t ldgt, int64_t n,
+                                                           int64_t ck, double *__restrict__ Wp, double *__restrict__ P,
+                                                           int64_t ldp, int direct)
+{
+    int u = (int)blockIdx.x, ti = 0, nch = trmm_chunks(n, 0, ck);
+    while (u >= (ti + 1) * nch) {  // uniform: item -> tile row
+        u -= (ti + 1) * nch;
+        ++ti;
+        nch = trmm_chunks(n, ti, ck);
+    }
+    const int tj = u / nch, ch = u % nch;
+    const int w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    const int64_t klo = (int64_t)ti * 128 + ch * ck, khi = (klo + ck) < n ? (klo + ck) : n;
+    const int64_t i0w = (int64_t)ti * 128 + wr * 64;  // Lt rows i >= i0w are zero for k < i0w
+    const int kstart = (int)(klo > i0w ? klo : i0w);
+    RowTile64 t;
+    t.zero();
+    if (khi > kstart && (khi - kstart) % 16 == 0) {  // WaveTile's scalar-base loads (depth a multiple of 16)
+        WaveTile<4, 4> wt;
+        wt.zero();
+        wt.setup(Lt + (int64_t)kstart * ldlt, ldlt, ti * 128 + wr * 64, (int)n, Gt + (int64_t)kstart * ldgt, ldgt,
+                 tj * 128 + wc * 64, (int)n, (int)(khi - kstart));
+        wt.run();
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) t.acc[a][b] = wt.acc[a][b];
+    } else {
+        t.run(Lt, ldlt, ti * 128 + wr * 64, (int)n, Gt, ldgt, tj * 128 + wc * 64, (int)n, kstart, (int)khi);
+    }
+    if (direct)
+        t.template store<true>(P + (int64_t)ti * 128 + (int64_t)tj * 128 * ldp, ldp, wr * 64, wc * 64,
+                               (int)(n - ti * 128), (int)(n - tj * 128));
+    else
+        t.template store<true>(Wp + (size_t)blockIdx.x * 128 * 128, 128, wr * 64, wc * 64, 128, 128);
+}
+
+// P tile (ti, tj) = sum of its partials in chunk order; lower part only on diagonal tiles.
+__global__ __launch_bounds__(256) void trmm_reduce_kernel(const double *__restrict__ Wp, int64_t n, int64_t ck,
+                                                          double *__restrict__ P, int64_t ldp)
+{
     int ti, tj;
-    lower_tile(xcd_remap((int)blockIdx.x, T * (T + 1) / 2), ti, tj);
-    const int w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1, l = lane_id();
-    if (ti == tj && wr == 0 && wc == 1) return;
-    const int64_t i0 = (int64_t)ti * 128 + wr * 64, j0 = (int64_t)tj * 128 + wc * 64;
-    if (i0 >= n || j0 >= n) return;
-    int64_t prow[4], qrow[4];
-    const double *pa[4], *qb[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        prow[a] = i0 + 16 * a + (l & 15);
-        pa[a] = L + (prow[a] < n ? prow[a] : n - 1) * ldl;
-        qrow[a] = j0 + 16 * a + (l & 15);
-        qb[a] = G + (qrow[a] < n ? qrow[a] : n - 1) * ldg;
+    lower_tile((int)blockIdx.x, ti, tj);
+    int64_t base = 0;
+    for (int t = 0; t < ti; ++t) base += (int64_t)(t + 1) * trmm_chunks(n, t, ck);
+    const int nch = trmm_chunks(n, ti, ck);
+    const double *src = Wp + (size_t)(base + (int64_t)tj * nch) * 128 * 128;
+    for (int e = threadIdx.x; e < 128 * 128; e += 256) {
+        const int r = e & 127, c = e >> 7;
+        const int64_t i = (int64_t)ti * 128 + r, j = (int64_t)tj * 128 + c;
+        if (i >= n || j >= n || j > i) continue;
+        double v = 0.0;
+        for (int ch = 0; ch < nch; ++ch) v += src[(size_t)ch * 128 * 128 + e];
+        P[i + j * ldp] = v;
     }
-    dvec4 acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
-    const int64_t kb = i0;  // rows i >= i0 have L[k][i] = 0 for k < i0
-    const int64_t S = (n - kb + 3) / 4;
-    constexpr int U = 4;
-    double pbuf[U][4], qbuf[U][4];
-    auto load = [&](int u, int64_t s) {
-        int64_t k = kb + 4 * (s < S ? s : S - 1) + (l >> 4);
-        k = k < n ? k : n - 1;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            pbuf[u][a] = pa[a][k];
-            qbuf[u][a] = qb[a][k];
-        }
-    };
-    auto mma = [&](int u, int64_t s) {
-        const int64_t k = kb + 4 * s + (l >> 4);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const double pv = (k < n && k >= prow[a]) ? pbuf[u][a] : 0.0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const double qv = (k < n && k >= qrow[b]) ? qbuf[u][b] : 0.0;
-                acc[a][b] = mfma_f64(qv, pv, acc[a][b]);
-            }
-        }
-    };
-    const int64_t Sp = (S + U - 1) / U * U;
-#pragma unroll
-    for (int u = 0; u < U - 1; ++u) load(u, u);
-    for (int64_t s0 = 0; s0 < Sp; s0 += U) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            load((u + U - 1) % U, s0 + u + U - 1);
-            mma(u, s0 + u);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t i = prow[a], j = j0 + 16 * b + 4 * r + (l >> 4);
-                if (i < n && j < n && j <= i) P[i + j * ldp] = acc[a][b][r];
-            }
 }
 
 // Full n x n outputs from a square input through 32 x 32 LDS tiles:
 //   MODE 0: out[i][j] = in[max(i,j)][min(i,j)] / 2   (symmetric completion of tril(in), halved)
 //   MODE 1: out = in^T
 //   MODE 2: out = (in + in^T) / 2
+//   MODE 3: as MODE 2 inside the 128 x 128 diagonal blocks, in[min(i,j)][max(i,j)] outside
+//           (in is symmetric and only its block-upper part was computed)
+//   MODE 4: out = tril(in)^T  (out[i][j] = in[j][i] for j >= i, else 0)
 template <int MODE>
 __global__ __launch_bounds__(256) void sym_tile_kernel(const double *__restrict__ in, int64_t ldi, int64_t n,
                                                        double *__restrict__ out, int64_t ldo)
@@ -116,6 +122,8 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(const double *__restrict_
         double v;
         if (MODE == 0) v = 0.5 * (i >= j ? in[i + j * ldi] : tr);
         else if (MODE == 1) v = tr;
+        else if (MODE == 3 && (i >> 7) != (j >> 7)) v = i < j ? in[i + j * ldi] : tr;
+        else if (MODE == 4) v = j >= i ? tr : 0.0;
         else v = 0.5 * (in[i + j * ldi] + tr);
         out[i + j * ldo] = v;
     }
@@ -157,12 +165,34 @@ __global__ __launch_bounds__(256) void rank2_lower_kernel(double *__restrict__ L
     Lb[i + j * ldlb] -= s;
 }
 
-void launch_trmm_tn_lower(const double *L, int64_t ldl, const double *G, int64_t ldg, int64_t n, double *P,
-                          int64_t ldp, hipStream_t s)
+// Lt = tril(L)^T, Gt = tril(G)^T; Wp: n*n doubles of scratch for the partials.
+void launch_trmm_tn_lower(const double *Lt, int64_t ldlt, const double *Gt, int64_t ldgt, int64_t n, double *P,
+                          int64_t ldp, double *Wp, hipStream_t s)
 {
     const int T = (int)((n + 127) / 128);
-    hipLaunchKernelGGL(trmm_tn_lower_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, L, ldl, G, ldg, n,
-                       P, ldp);
+    auto count = [&](int64_t ck) {
+        int64_t items = 0;
+        for (int t = 0; t < T; ++t) items += (int64_t)(t + 1) * ((n - 128 * (int64_t)t + ck - 1) / ck);
+        return items;
+    };
+    int C = (T + 2) / 3 > 1 ? (T + 2) / 3 : 1;  // chunk of 128 C >= n/3
+    int64_t items = count(128 * (int64_t)C);
+    while (items * 128 * 128 > n * n && C < T) items = count(128 * (int64_t)(++C));
+    const int direct = items * 128 * 128 > n * n ? 1 : 0;  // then C == T: one chunk per tile
+    const int64_t ck = 128 * (int64_t)C;
+    if (direct) {
+        // P may be Gt's buffer, still read by other workgroups of the launch: the tiles go to
+        // Wp (as an n x n matrix) and are copied over afterwards (small n only)
+        hipLaunchKernelGGL(trmm_part_kernel, dim3((unsigned)items), dim3(256), 0, s, Lt, ldlt, Gt, ldgt, n, ck,
+                           (double *)nullptr, Wp, n, 1);
+        (void)hipMemcpy2DAsync(P, (size_t)ldp * sizeof(double), Wp, (size_t)n * sizeof(double),
+                               (size_t)n * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, s);
+    } else {
+        hipLaunchKernelGGL(trmm_part_kernel, dim3((unsigned)items), dim3(256), 0, s, Lt, ldlt, Gt, ldgt, n, ck, Wp,
+                           (double *)nullptr, (int64_t)0, 0);
+        hipLaunchKernelGGL(trmm_reduce_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, Wp, n, ck, P,
+                           ldp);
+    }
 }
 
 void launch_sym_tile(int mode, const double *in, int64_t ldi, int64_t n, double *out, int64_t ldo, hipStream_t s)
@@ -172,8 +202,12 @@ void launch_sym_tile(int mode, const double *in, int64_t ldi, int64_t n, double 
         hipLaunchKernelGGL(sym_tile_kernel<0>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
     else if (mode == 1)
         hipLaunchKernelGGL(sym_tile_kernel<1>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
-    else
+    else if (mode == 2)
         hipLaunchKernelGGL(sym_tile_kernel<2>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
+    else if (mode == 3)
+        hipLaunchKernelGGL(sym_tile_kernel<3>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
+    else
+        hipLaunchKernelGGL(sym_tile_kernel<4>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
 }
 
 void launch_solve_adjoint_update(const double *L, int64_t ldl, int64_t n, int64_t r, const double *X, int64_t ldx,

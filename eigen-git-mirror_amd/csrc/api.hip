@@ -724,21 +724,26 @@ int mi_dpotrf_adjoint(mi_llt_handle h, char uplo, int64_t n, const double *dL, i
     const size_t leaf = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double));
     const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
     const size_t sq = round_up((size_t)n * (size_t)n * sizeof(double));
-    int rc = reserve(h, leaf + inv + 2 * sq);
+    int rc = reserve(h, leaf + inv + 3 * sq);
     if (rc) return rc;
     double *wleaf = (double *)h->ws;
     double *winv = (double *)((char *)h->ws + leaf);
     double *W2 = (double *)((char *)h->ws + leaf + inv);
     double *W3 = (double *)((char *)h->ws + leaf + inv + sq);
+    double *Wt = (double *)((char *)h->ws + leaf + inv + 2 * sq);  // tril(L)^T
     const double nd = (double)n;
-    traced(h, s, MI_TRACE_GEMM, nd * nd * nd / 3.0, [&] { mi::launch_trmm_tn_lower(dL, ldl, dG, ldg, n, W3, n, s); });
-    mi::launch_sym_tile(0, W3, n, n, W2, n, s);           // M
+    mi::launch_sym_tile(4, dL, ldl, n, Wt, n, s);  // tril(L)^T
+    mi::launch_sym_tile(4, dG, ldg, n, W3, n, s);  // tril(Lbar)^T, then overwritten by P
+    traced(h, s, MI_TRACE_GEMM, nd * nd * nd / 3.0, [&] { mi::launch_trmm_tn_lower(Wt, n, W3, n, n, W3, n, W2, s); });
+    mi::launch_sym_tile(0, W3, n, n, W2, n, s);              // M
     mi::launch_leaf_inverses<double>(dL, ldl, n, wleaf, s);  // recomputed from L
     block_inverses<double>(h, dL, ldl, n, wleaf, winv, nullptr);
-    if ((rc = solve_only<double>(h, dL, ldl, n, W2, n, n, winv, nullptr, 2))) return rc;  // Z = L^{-T} M
-    mi::launch_sym_tile(1, W2, n, n, W3, n, s);           // Z^T
-    if ((rc = solve_only<double>(h, dL, ldl, n, W3, n, n, winv, nullptr, 2))) return rc;  // S = L^{-T} Z^T
-    mi::launch_sym_tile(2, W3, n, n, dG, ldg, s);         // Abar = (S + S^T) / 2
+    traced(h, s, MI_TRACE_SOLVE, nd * nd * nd,
+           [&] { mi::launch_trsm_xl_multi(Wt, n, n, W2, n, n, winv, false, s); });  // Y = M L^{-1}
+    mi::launch_sym_tile(1, W2, n, n, W3, n, s);                                      // Y^T = L^{-T} M
+    traced(h, s, MI_TRACE_SOLVE, nd * nd * nd / 2.0,
+           [&] { mi::launch_trsm_xl_multi(Wt, n, n, W3, n, n, winv, true, s); });   // S = Y^T L^{-1}, block-upper
+    mi::launch_sym_tile(3, W3, n, n, dG, ldg, s);                                    // Abar = sym(S)
     return check_launch(h, "potrf_adjoint");
 }
 

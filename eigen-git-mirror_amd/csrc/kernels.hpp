@@ -79,11 +79,16 @@ void launch_vbatched(int64_t batch, const int *d_n, T *const *d_A, const int *d_
                      int group, int lpt, hipStream_t s);
 
 // Reverse mode (adjoint.hip).
-// P (lower) = tril(L^T G) for lower L, G.
-void launch_trmm_tn_lower(const double *L, int64_t ldl, const double *G, int64_t ldg, int64_t n, double *P,
-                          int64_t ldp, hipStream_t s);
+// P (lower) = tril(L^T G) for lower L, G, from Lt = tril(L)^T and Gt = tril(G)^T; Wp: n*n
+// doubles of scratch (the k-chunk partials).  P may be Gt's buffer.
+void launch_trmm_tn_lower(const double *Lt, int64_t ldlt, const double *Gt, int64_t ldgt, int64_t n, double *P,
+                          int64_t ldp, double *Wp, hipStream_t s);
 // mode 0: out = symmetric completion of tril(in), halved; 1: out = in^T; 2: out = (in + in^T)/2.
 void launch_sym_tile(int mode, const double *in, int64_t ldi, int64_t n, double *out, int64_t ldo, hipStream_t s);
+// B (nrow x n) <- B L^{-1} by blocked right-side substitution, one launch per 128-column
+// block, with Lt = L^T; upper: only the block-upper tiles (nrow == n, the result symmetric).
+void launch_trsm_xl_multi(const double *Lt, int64_t ldlt, int64_t n, double *B, int64_t ldb, int64_t nrow,
+                          const double *Winv, bool upper, hipStream_t s);
 // Lbar (lower) -= tril(Bb (L^T X)^T + X (L^T Bb)^T); U, V: n x r scratch.
 void launch_solve_adjoint_update(const double *L, int64_t ldl, int64_t n, int64_t r, const double *X, int64_t ldx,
                                  const double *Bb, int64_t ldbb, double *Lb, int64_t ldlb, double *U, double *V,

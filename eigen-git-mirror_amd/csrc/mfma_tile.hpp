@@ -376,6 +376,104 @@ struct TileSel<float, PREC> {
     using type = WaveTileF<4, 4, PREC>;
 };
 
+// A second FP64 wave tile, for irregular k ranges: TA x TB MFMA tiles, acc[a][b] = tile
+// (16a.., 16b..); row-contiguous operands (element (row r, k) at base + r + k * ld).  Rows >=
+// rmax are clamped (their results are never stored); k outside [klo, kmax) reads as 0 (klo a
+// multiple of 4), so any depth works (WaveTile needs a multiple of 4 k-steps per buffer
+// round).  U k-steps of loads in flight in a rolling register buffer.
+template <int TA, int TB>
+struct RowTile {
+    dvec4 acc[TA][TB];
+    __device__ __forceinline__ void zero()
+    {
+#pragma unroll
+        for (int a = 0; a < TA; ++a)
+#pragma unroll
+            for (int b = 0; b < TB; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
+    }
+    // acc += P Q^T over k in [klo, kmax), P rows p0.., Q rows q0..
+    __device__ __forceinline__ void run(const double *P, int64_t ldp, int p0, int pmax, const double *Q, int64_t ldq,
+                                        int q0, int qmax, int klo, int kmax)
+    {
+        const int l = lane_id(), kl = l >> 4;
+        const double *pa[TA], *qb[TB];
+#pragma unroll
+        for (int a = 0; a < TA; ++a) {
+            int r = p0 + 16 * a + (l & 15);
+            r = r < pmax ? r : pmax - 1;
+            pa[a] = P + r;
+        }
+#pragma unroll
+        for (int b = 0; b < TB; ++b) {
+            int q = q0 + 16 * b + (l & 15);
+            q = q < qmax ? q : qmax - 1;
+            qb[b] = Q + q;
+        }
+        const int s0 = klo >> 2, S = (kmax + 3) >> 2;  // k-steps [s0, S)
+        if (S <= s0) return;
+        constexpr int U = 4;
+        double pbuf[U][TA], qbuf[U][TB];
+        auto load = [&](int u, int s) {
+            int k = 4 * (s < S ? s : S - 1) + kl;
+            k = k < kmax ? k : kmax - 1;
+#pragma unroll
+            for (int a = 0; a < TA; ++a) pbuf[u][a] = pa[a][(int64_t)k * ldp];
+#pragma unroll
+            for (int b = 0; b < TB; ++b) qbuf[u][b] = qb[b][(int64_t)k * ldq];
+        };
+        auto mma = [&](int u, int s) {
+            const bool ok = 4 * s + kl < kmax;
+#pragma unroll
+            for (int a = 0; a < TA; ++a) {
+                const double pv = ok ? pbuf[u][a] : 0.0;
+#pragma unroll
+                for (int b = 0; b < TB; ++b) acc[a][b] = mfma_f64(qbuf[u][b], pv, acc[a][b]);
+            }
+        };
+        const int Sp = s0 + (S - s0 + U - 1) / U * U;
+#pragma unroll
+        for (int u = 0; u < U - 1; ++u) load(u, s0 + u);
+        for (int sb = s0; sb < Sp; sb += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                load((u + U - 1) % U, sb + u + U - 1);
+                if (sb + u < S) mma(u, sb + u);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    // C[i][j] (column-major, rows i0.., cols j0..) = acc (ASSIGN) or -= acc, i < imax, j < jmax.
+    template <bool ASSIGN>
+    __device__ __forceinline__ void store(double *C, int64_t ldc, int i0, int j0, int imax, int jmax) const
+    {
+        const int l = lane_id();
+#pragma unroll
+        for (int a = 0; a < TA; ++a) {
+            const int i = i0 + 16 * a + (l & 15);
+            double old[TB][4];
+            if (!ASSIGN) {
+#pragma unroll
+                for (int b = 0; b < TB; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = j0 + 16 * b + 4 * r + (l >> 4);
+                        old[b][r] = C[(int64_t)(i < imax ? i : imax - 1) + (int64_t)(j < jmax ? j : jmax - 1) * ldc];
+                    }
+            }
+            if (i < imax) {
+#pragma unroll
+                for (int b = 0; b < TB; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = j0 + 16 * b + 4 * r + (l >> 4);
+                        if (j < jmax) C[i + (int64_t)j * ldc] = ASSIGN ? acc[a][b][r] : old[b][r] - acc[a][b][r];
+                    }
+            }
+        }
+    }
+};
+using RowTile64 = RowTile<4, 4>;
+
 // Workgroup -> tile remap that gives each XCD a contiguous range of the tile sequence
 // (workgroups are dispatched round-robin over the 8 XCDs; each XCD has its own L2).
 __device__ __forceinline__ int xcd_remap(int w, int ntiles)

@@ -19,7 +19,7 @@ def spd(n, seed, dev):
     return torch.from_numpy(workloads.gram_spd(n, seed=seed)).to(dev)
 
 
-@pytest.mark.parametrize("n", [1, 5, 37, 130, 300])
+@pytest.mark.parametrize("n", [1, 5, 37, 130, 300, 384, 1000])
 def test_cholesky_adjoint_matches_torch(dev, n):
     print(f"[gpu] adjoint vs torch n={n}", flush=True)
     a = spd(n, 7, dev)
