@@ -156,14 +156,17 @@ class Handle:
         return out
 
 
-_default: Handle | None = None
+_handles: dict[int, Handle] = {}
 
 
 def default_handle() -> Handle:
-    global _default
-    if _default is None:
-        _default = Handle()
-    return _default.bind_stream()
+    """The handle of torch's current stream (one per stream: a handle's workspace, tickets and
+    flags serve one stream's calls in order, so work on several streams needs several)."""
+    s = torch.cuda.current_stream()
+    h = _handles.get(s.cuda_stream)
+    if h is None:
+        h = _handles[s.cuda_stream] = Handle()
+    return h.bind_stream(s)
 
 
 def _uplo(u: str) -> bytes:
