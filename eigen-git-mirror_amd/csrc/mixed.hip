@@ -68,33 +68,66 @@ __global__ __launch_bounds__(256) void sqrt_sum_kernel(const double *__restrict_
 
 // Partial products of A X, A symmetric in one triangle of FP64 storage: workgroup
 // (row tile of 64, group of 16 right-hand sides, k-chunk) -> part[chunk] (n x nrhs, ld n).
-// Each wave owns 16 rows; lane (i = l%16, k = l/16) reads A_sym(i, k) from the stored
-// triangle, X is the MFMA's other operand.  8 k-steps of loads are issued before their MFMAs.
+// Each wave owns 16 rows; lane (i = l%16, k = l/16) supplies A_sym(i, k), X is the MFMA's other
+// operand; 32 k (8 MFMA steps) per group, all loads of a group issued before its MFMAs.
+// Where the group's k lie on the stored side of the row tile (k <= i for the lower triangle),
+// A(i, k) is read in place -- lanes over consecutive i, coalesced.  Where they all lie on the
+// other side, A_sym(i, k) = A(k, i) is stored with k contiguous: the workgroup reads the 32 x 64
+// piece with lanes over k (coalesced) into LDS and the waves take their operand from there
+// (reading it in place would stride by lda across the lanes).  The groups that straddle the
+// diagonal pick per element.
 __global__ __launch_bounds__(256) void symm_part_kernel(const double *__restrict__ A, int64_t lda, int64_t n,
                                                         int upper, const double *__restrict__ X, int64_t ldx,
                                                         int64_t nrhs, int64_t kchunk, double *__restrict__ part)
 {
+    __shared__ double Ts[33 * 64];  // Ts[kk + 33 * ii]: 32 k (padded to 33) x 64 rows of the tile
     const int l = lane_id(), w = threadIdx.x >> 6;
-    const int64_t i = (int64_t)blockIdx.x * 64 + 16 * w + (l & 15);
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int64_t i = r0 + 16 * w + (l & 15);
     const int64_t c0 = (int64_t)blockIdx.y * 16;
     const int64_t kb = (int64_t)blockIdx.z * kchunk;
     const int64_t ke = (kb + kchunk) < n ? (kb + kchunk) : n;
     const int64_t ic = i < n ? i : n - 1;
     const int64_t cj = c0 + (l & 15);
     const int64_t cjc = cj < nrhs ? cj : nrhs - 1;
+    const int64_t rlast = (r0 + 63) < n ? r0 + 63 : n - 1;  // last valid row of the tile
     dvec4 acc = {0.0, 0.0, 0.0, 0.0};
     constexpr int U = 8;
     for (int64_t k0 = kb; k0 < ke; k0 += 4 * U) {
+        const int64_t kl = (k0 + 4 * U - 1) < ke ? k0 + 4 * U - 1 : ke - 1;  // last k of the group
+        // the group lies entirely on the unstored side of the whole row tile: transposed read
+        const bool transposed = upper ? (kl < r0) : (k0 > rlast);
         double av[U], xv[U];
         bool ok[U];
+        if (transposed) {  // uniform over the workgroup
+            // Ts[kk + 33 * ii] = A_sym(r0 + ii, k0 + kk) = A(k0 + kk, r0 + ii), stored (for either
+            // triangle) at A[k + r * lda]: lanes over kk, coalesced
+            for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+                const int kk = e & 31, ii = e >> 5;
+                const int64_t k = k0 + kk, r = r0 + ii;
+                const int64_t kc2 = k < ke ? k : ke - 1, rc2 = r < n ? r : n - 1;
+                Ts[kk + 33 * ii] = A[kc2 + rc2 * lda];
+            }
+            __syncthreads();
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t k = k0 + 4 * u + (l >> 4);
-            ok[u] = k < ke;
-            const int64_t kc = ok[u] ? k : kb;
-            const bool low = upper ? (ic <= kc) : (ic >= kc);  // (ic, kc) is in the stored triangle
-            av[u] = low ? A[ic + kc * lda] : A[kc + ic * lda];
-            xv[u] = X[kc + cjc * ldx];
+            for (int u = 0; u < U; ++u) {
+                const int kk = 4 * u + (l >> 4);
+                const int64_t k = k0 + kk;
+                ok[u] = k < ke;
+                av[u] = Ts[kk + 33 * (16 * w + (l & 15))];
+                xv[u] = X[(ok[u] ? k : kb) + cjc * ldx];
+            }
+            __syncthreads();  // Ts is rewritten by the next transposed group
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t k = k0 + 4 * u + (l >> 4);
+                ok[u] = k < ke;
+                const int64_t kc = ok[u] ? k : kb;
+                const bool low = upper ? (ic <= kc) : (ic >= kc);  // (ic, kc) is in the stored triangle
+                av[u] = low ? A[ic + kc * lda] : A[kc + ic * lda];
+                xv[u] = X[kc + cjc * ldx];
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) acc = mfma_f64(ok[u] ? xv[u] : 0.0, av[u], acc);
