@@ -25,10 +25,11 @@ struct mi_llt_context {
     void *ws = nullptr;       // grow-only device workspace
     size_t ws_bytes = 0;
     int *d_info = nullptr;    // device int for the synchronous entry points
-    // persistent synchronisation words of the solve: [status | 2 tickets | flags...]
+    // persistent synchronisation words: [status | 2 solve tickets | - | factor tickets | panel flag | ...]
     uint32_t *sync = nullptr;
     size_t sync_words = 0;
-    uint32_t epoch = 0;          // solve flags
+    double *ypk = nullptr;       // the solve's packed copies of finished blocks (grow-only)
+    size_t ypk_bytes = 0;
     uint32_t panel_epoch = 0;    // panel-step flag
     uint32_t panel_tickets = 0;  // tickets drawn so far from the panel ticket counter
     char err[256] = {0};
@@ -178,7 +179,8 @@ double panel_flops(int64_t n, int64_t k0, int64_t NB)
 }
 
 template <typename T>
-int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, int *d_info, int prec = MI_PREC_FP64)
+int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, int *d_info, int prec = MI_PREC_FP64,
+                 __bf16 *wsplit = nullptr)
 {
     constexpr int64_t NB = mi::kNB;
     const int64_t nblk = (n + NB - 1) / NB;
@@ -206,7 +208,7 @@ int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, i
         uint32_t grid = 0;
         traced(h, s, MI_TRACE_POTRF, fl, [&] {
             mi::launch_factor_step<T>(A, lda, n, kp, k, wleaf, d_info, h->sync + 4, h->panel_tickets, h->sync + 5,
-                                      epoch, (int *)h->sync, s, &grid, prec);
+                                      epoch, (int *)h->sync, s, &grid, prec, wsplit);
         });
         h->panel_tickets += grid;
     }
@@ -227,20 +229,27 @@ template <typename T>
 int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                const double *winv, const int *d_info, int sweeps = 3)
 {
-    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
-    const int64_t ngrp = (nrhs + 15) / 16;
-    int rc = reserve_sync(h, (size_t)(nblk * ngrp));
+    int rc = reserve_sync(h, 0);
     if (rc) return rc;
-    if (h->epoch > 0x7fffffffu) {  // epochs only grow: start over on fresh (zero) flags
-        hipError_t e = hipMemsetAsync(h->sync + kSyncHeader, 0, (h->sync_words - kSyncHeader) * 4, h->stream);
-        if (e != hipSuccess) return fail_hip(h, e, "hipMemsetAsync");
-        h->epoch = 0;
+    const size_t pk = 2 * (size_t)mi::solve_pack_doubles(n, nrhs) * sizeof(double);  // forward, backward
+    if (pk > h->ypk_bytes) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
+        if (h->ypk) hipFree(h->ypk);
+        h->ypk = nullptr;
+        h->ypk_bytes = 0;
+        e = hipMalloc(&h->ypk, pk);
+        if (e != hipSuccess) {
+            h->ypk = nullptr;
+            snprintf(h->err, sizeof h->err, "solve scratch of %zu bytes: %s", pk, hipGetErrorString(e));
+            return MI_LLT_E_ALLOC;
+        }
+        h->ypk_bytes = pk;
     }
-    const uint32_t ef = ++h->epoch, eb = ++h->epoch;
     const double per_sweep = (double)n * (double)n * (double)nrhs;
     traced(h, h->stream, MI_TRACE_SOLVE, per_sweep * ((sweeps & 1) + ((sweeps >> 1) & 1)), [&] {
-        mi::launch_solve_persistent<T>(L, ldl, n, B, ldb, nrhs, winv, h->sync + kSyncHeader, (int *)h->sync + 1, ef,
-                                       eb, d_info, (int *)h->sync, h->stream, sweeps);
+        mi::launch_solve_persistent<T>(L, ldl, n, B, ldb, nrhs, winv, h->ypk, (int *)h->sync + 1, d_info, (int *)h->sync,
+                                       h->stream, sweeps);
     });
     return 0;
 }
@@ -254,13 +263,15 @@ int solve_lower(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, 
 }
 
 // Workspace of mi_dposv_mixed: [ W (FP32, or FP64 for the plain path) | Wleaf | Winv | R | D |
-// residual partials | norm partials | scalars: anorm, berr, rr[nrhs], xx[nrhs] ]
+// residual partials | norm partials | scalars: anorm, berr, rr[nrhs], xx[nrhs] | split-panel
+// ring (FP32 path) ]
 struct MixedLayout {
     float *w32 = nullptr;
     double *w64 = nullptr;
     int64_t ldw = 0;
     double *wleaf = nullptr, *winv = nullptr, *R = nullptr, *D = nullptr, *part = nullptr, *tpart = nullptr;
     double *anorm = nullptr, *berr = nullptr, *rr = nullptr, *xx = nullptr;
+    __bf16 *split = nullptr;
 };
 
 int prepare_mixed(mi_llt_handle h, int64_t n, int64_t nrhs, bool fp64, MixedLayout &m)
@@ -273,7 +284,8 @@ int prepare_mixed(mi_llt_handle h, int64_t n, int64_t nrhs, bool fp64, MixedLayo
     const size_t part = round_up((size_t)mi::residual_chunks(n, nrhs) * (size_t)n * (size_t)nrhs * sizeof(double));
     const size_t tpart = round_up((size_t)mi::sym_tiles64(n) * sizeof(double));
     const size_t scal = round_up((size_t)(2 + 2 * nrhs) * sizeof(double));
-    int rc = reserve(h, wbytes + leaf + inv + 2 * vec + part + tpart + scal);
+    const size_t split = fp64 ? 0 : round_up((size_t)(4 * mi::split_slot_elems(n)) * sizeof(__bf16));
+    int rc = reserve(h, wbytes + leaf + inv + 2 * vec + part + tpart + scal + split);
     if (rc) return rc;
     char *p = (char *)h->ws;
     m = MixedLayout();
@@ -299,6 +311,8 @@ int prepare_mixed(mi_llt_handle h, int64_t n, int64_t nrhs, bool fp64, MixedLayo
     m.berr = m.anorm + 1;
     m.rr = m.anorm + 2;
     m.xx = m.rr + nrhs;
+    p += scal;
+    if (!fp64) m.split = (__bf16 *)p;
     return 0;
 }
 
@@ -384,6 +398,7 @@ int mi_llt_destroy(mi_llt_handle h)
     if (!valid(h)) return MI_LLT_E_HANDLE;
     hipStreamSynchronize(h->stream);
     if (h->sync) hipFree(h->sync);
+    if (h->ypk) hipFree(h->ypk);
     if (h->vb_ws) hipFree(h->vb_ws);
     if (h->ws) hipFree(h->ws);
     if (h->d_info) hipFree(h->d_info);
@@ -650,7 +665,7 @@ int mi_dposv_mixed(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const do
     if ((rc = prepare_mixed(h, n, nrhs, false, m))) return rc;
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, h->d_info, 0);
     mi::launch_sym_copy_lower<float>(dA, lda, n, upper, m.w32, m.ldw, m.tpart, m.anorm, s);
-    if ((rc = factor_lower<float>(h, m.w32, m.ldw, n, m.wleaf, h->d_info, prec))) return rc;
+    if ((rc = factor_lower<float>(h, m.w32, m.ldw, n, m.wleaf, h->d_info, prec, m.split))) return rc;
     int inf = 0;
     if ((rc = read_back(h, &inf, h->d_info, sizeof(int)))) return rc;
     if (inf != 0) return fp64_path(-1);  // not positive definite in the low-precision factor

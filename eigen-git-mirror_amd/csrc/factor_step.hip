@@ -42,6 +42,9 @@ __device__ __forceinline__ void pair_tile(int u, int c0, int nblk, int &i, int &
 //     tiles (0: none); slab: first column of the K-wide operand (the last one or two panels).
 // S: storage type (double, or float for the mixed-precision factorization), PREC: the
 // trailing-update arithmetic (kPrecF64 with double; FP32/BF16/split BF16 MFMA with float).
+// Wsplit (split BF16 arithmetic only, else null): a ring of 4 panel slots of slot_elems BF16
+// each (hi parts of n rows x 128, then the lo parts); the row roles of panel q write slot q % 4
+// and the trailing update reads its depth from the slots of the slab's panels.
 // Workgroups per CU the kernel is compiled for: 2 fits two 72 KiB LDS blocks and 256 VGPRs
 // per lane on a CU (the second workgroup hides the first one's barrier and memory waits).
 #ifndef MI_FACTOR_WG_PER_CU
@@ -53,7 +56,7 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
                                                           int Gp, int c0, int nsyrk, int64_t slab,
                                                           double *__restrict__ Wleaf, int *info, uint32_t *tickets,
                                                           uint32_t ticket_base, uint32_t *flag, uint32_t epoch,
-                                                          int *status)
+                                                          int *status, __bf16 *Wsplit, int64_t slot_elems)
 {
     __shared__ __align__(16) double smem[kSmem];
     __shared__ int s_role, s_ok;
@@ -63,8 +66,9 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
     if (factor_failed(info)) return;
     const int t = __builtin_amdgcn_readfirstlane(s_role);  // uniform: every derived address is scalar
     if (Gp > 0 && (t == 0 || t > nsyrk)) {  // ticket 0: diagonal role; last Gp-1: row roles
+        __bf16 *split = Wsplit ? Wsplit + ((kp / kNB) & 3) * slot_elems : nullptr;
         panel_role<S, PREC>(t == 0 ? 0 : t - nsyrk, smem, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch,
-                            status);
+                            status, split, slot_elems / 2);
         return;
     }
     const int u = Gp > 0 ? t - 1 : t;
@@ -73,6 +77,16 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
     pair_tile(xcd_remap(u, nsyrk), c0, nblk, i, c);
     const int64_t r = (int64_t)c * kNB;
     const int64_t m = n - r;
+    if constexpr (sizeof(S) == 4 && PREC == kPrecBF16x3) {
+        if (Wsplit) {
+            const int64_t sp = slab / kNB;  // first panel of the slab
+            const __bf16 *S0 = Wsplit + (sp & 3) * slot_elems;
+            const __bf16 *S1 = Wsplit + ((sp + 1) & 3) * slot_elems;
+            syrk_tile_split<K>(A + r + r * lda, lda, S0, S1, slot_elems / 2, (int)r, (int)m, (int)(m < kNB ? m : kNB),
+                               i - c, 0);
+            return;
+        }
+    }
     syrk_tile<K, S, PREC>(A + r + r * lda, lda, A + r + slab * lda, lda, (int)m, (int)(m < kNB ? m : kNB), i - c,
                           0);
 }
@@ -92,20 +106,20 @@ int pair_tiles(int64_t n, int64_t k, int *c0_out)
 template <typename S, int PREC>
 static void launch_prec(dim3 grid, hipStream_t s, int K, S *A, int64_t lda, int64_t n, int64_t kp, int Gp, int c0,
                         int nsyrk, int64_t slab, double *Wleaf, int *info, uint32_t *tickets, uint32_t ticket_base,
-                        uint32_t *flag, uint32_t epoch, int *status)
+                        uint32_t *flag, uint32_t epoch, int *status, __bf16 *Wsplit, int64_t slot_elems)
 {
     if (K == 256)
         hipLaunchKernelGGL((factor_step_kernel<S, PREC, 256>), grid, dim3(256), 0, s, A, lda, n, kp, Gp, c0, nsyrk,
-                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status);
+                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status, Wsplit, slot_elems);
     else
         hipLaunchKernelGGL((factor_step_kernel<S, PREC, 128>), grid, dim3(256), 0, s, A, lda, n, kp, Gp, c0, nsyrk,
-                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status);
+                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status, Wsplit, slot_elems);
 }
 
 template <typename S>
 void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
-                        hipStream_t s, uint32_t *grid_out, int prec)
+                        hipStream_t s, uint32_t *grid_out, int prec, __bf16 *Wsplit)
 {
     const int Gp = kp >= 0 ? (int)((n - kp + 127) / 128) : 0;
     int c0 = 0;
@@ -116,24 +130,27 @@ void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, dou
     *grid_out = grid;
     if (grid == 0) return;
     const dim3 g(grid);
+    const int64_t slot = split_slot_elems(n);
     if constexpr (sizeof(S) == 8) {
         launch_prec<S, kPrecF64>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base, flag,
-                                 epoch, status);
+                                 epoch, status, nullptr, slot);
     } else if (prec == kPrecBF16) {
         launch_prec<S, kPrecBF16>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
-                                  flag, epoch, status);
+                                  flag, epoch, status, nullptr, slot);
     } else if (prec == kPrecBF16x3) {
         launch_prec<S, kPrecBF16x3>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
-                                    flag, epoch, status);
+                                    flag, epoch, status, Wsplit, slot);
     } else {
         launch_prec<S, kPrecF32>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base, flag,
-                                 epoch, status);
+                                 epoch, status, nullptr, slot);
     }
 }
 
 template void launch_factor_step<double>(double *, int64_t, int64_t, int64_t, int64_t, double *, int *, uint32_t *,
-                                         uint32_t, uint32_t *, uint32_t, int *, hipStream_t, uint32_t *, int);
+                                         uint32_t, uint32_t *, uint32_t, int *, hipStream_t, uint32_t *, int,
+                                         __bf16 *);
 template void launch_factor_step<float>(float *, int64_t, int64_t, int64_t, int64_t, double *, int *, uint32_t *,
-                                        uint32_t, uint32_t *, uint32_t, int *, hipStream_t, uint32_t *, int);
+                                        uint32_t, uint32_t *, uint32_t, int *, hipStream_t, uint32_t *, int,
+                                         __bf16 *);
 
 }  // namespace mi

@@ -18,7 +18,10 @@ constexpr int kNB = 128;
 //   Winv[b] : L_bb^{-T}, 128 x 128, ld 128 (built for the solve)
 constexpr int64_t kPanelDoubles = 36 * 256;
 constexpr int64_t kLeafOffset = 28 * 256;
-constexpr int64_t kInvDoubles = 2 * 128 * 128;  // X_b^T then X_b, each 128 x 128, ld 128
+// Per diagonal block b, four 128 x 128 matrices (ld 128): X_b^T, X_b (X_b = L_bb^{-1}), then the
+// couplings of the solve M_b = X_b L_{b,b-1} and N_b = X_b^T L_{b+1,b}^T (zero where undefined).
+constexpr int64_t kInvDoubles = 4 * 128 * 128;
+constexpr int64_t kInvX = 128 * 128, kInvM = 2 * 128 * 128, kInvN = 3 * 128 * 128;  // offsets
 
 // Device status bits (a word in the handle's persistent device memory).
 constexpr int MI_SOLVE_TIMEOUT = 1;
@@ -29,25 +32,33 @@ constexpr int MI_PANEL_TIMEOUT = 2;
 // size (= tickets drawn) to *grid_out.  tickets/flag: persistent words of the handle,
 // ticket_base = number of tickets drawn before this launch, epoch = this step's flag value.
 // S = double: FP64 throughout (prec ignored); S = float: FP32 storage, FP64 diagonal factor
-// and panel solve, trailing updates in the MFMA arithmetic `prec` (kPrecF32/BF16/BF16x3).
+// and panel solve, trailing updates in the MFMA arithmetic `prec` (kPrecF32/BF16/BF16x3);
+// with kPrecBF16x3 and Wsplit given, the trailing update reads panels split once into BF16
+// hi/lo parts by the panel solve (4 * split_slot_elems(n) elements, no other contents needed).
 template <typename S>
 void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
-                        hipStream_t s, uint32_t *grid_out, int prec);
-// Winv[b] = {X_b^T, X_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).
+                        hipStream_t s, uint32_t *grid_out, int prec, __bf16 *Wsplit = nullptr);
+// BF16 elements of one slot of the split-panel ring (Wsplit, 4 slots) of the split-BF16
+// factorization: the hi parts of n rows (rounded up to 128) x 128 columns, then the lo parts.
+inline int64_t split_slot_elems(int64_t n) { return 2 * ((n + 127) / 128 * 128) * 128; }
+// Winv[b] = {X_b^T, X_b, M_b, N_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).
 template <typename T>
 void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv, const int *info,
                            hipStream_t s);
 // Wleaf from an already factored matrix (standalone solve).
 template <typename T>
 void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s);
-// Solve: forward then backward sweep in place on B (one persistent launch each).
-// flags: ceil(nrhs/16) * nblk words that persist across calls; tickets: 2 ints.
+// Solve: forward then backward sweep in place on B (one persistent launch each, each after a
+// small reset launch).  tickets: 2 ints; Ypk: 2 * solve_pack_doubles(n, nrhs) doubles of
+// scratch (each finished block's packed copy for its consumers, per sweep).
 // sweeps: 1 forward only (L Y = B), 2 backward only (L^T X = B), 3 both.
 template <typename T>
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
-                             const double *Winv, uint32_t *flags, int *tickets, uint32_t epoch_fwd,
-                             uint32_t epoch_bwd, const int *info, int *status, hipStream_t s, int sweeps = 3);
+                             const double *Winv, double *Ypk, int *tickets, const int *info, int *status, hipStream_t s,
+                             int sweeps = 3);
+// Doubles of one sweep's packed block copies for n rows and nrhs right-hand sides.
+inline int64_t solve_pack_doubles(int64_t n, int64_t nrhs) { return ((n + 127) / 128) * ((nrhs + 15) / 16) * 2048; }
 // Symmetric-storage helpers.
 void launch_upper_to_lower(const double *A, int64_t lda, int64_t n, double *W, int64_t ldw, hipStream_t s);
 void launch_lower_to_upper(const double *W, int64_t ldw, int64_t n, double *A, int64_t lda, hipStream_t s);

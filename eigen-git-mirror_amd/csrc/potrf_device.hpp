@@ -297,12 +297,14 @@ __device__ __forceinline__ void stage_panel_image(double *smem, const double *__
 // read); the result L^{-T} goes to O (ld 128) and its transpose L^{-1} to O + 128*128.  T =
 // float: P and L are read as FP32 and widened, the arithmetic is FP64, the result is rounded
 // back to FP32.  If IMAGE is given, the panel image is copied from there instead of built from
-// L and Xl.
+// L and Xl.  If SPLIT (a split-panel slot) is given (T = float), the rounded rows are also
+// written split into BF16 hi/lo parts, row i as row split_row0 + i of the slot (split_store_row).
 template <bool IDENTITY, typename T>
 __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64_t ldp, int m,
                                           const T *__restrict__ L, int64_t ldl, int bs,
                                           const double *__restrict__ Xl, double *__restrict__ O,
-                                          const double *__restrict__ image = nullptr)
+                                          const double *__restrict__ image = nullptr, __bf16 *split = nullptr,
+                                          int64_t split_lo = 0, int64_t split_row0 = 0)
 {
     const int l = lane_id(), w = threadIdx.x >> 6;
     const int row0 = 32 * w;
@@ -345,6 +347,16 @@ __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64
                         P[i + (int64_t)j * ldp] = (T)X[t][p][r];
                     }
                 }
+            if constexpr (!IDENTITY && sizeof(T) == 4) {
+                if (split) {
+                    float v[8][4];
+#pragma unroll
+                    for (int p = 0; p < 8; ++p)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) v[p][r] = (float)X[t][p][r];
+                    split_store_row(split, split_lo, split_row0 + i, v);
+                }
+            }
         }
     }
 }
@@ -474,7 +486,8 @@ __device__ __forceinline__ bool wait_panel_flag(int *s_ok, const uint32_t *flag,
 template <typename T, int PREC>
 __device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__restrict__ A,
                                            int64_t lda, int64_t n, int64_t k0, double *__restrict__ Wleaf, int *info,
-                                           uint32_t *flag, uint32_t epoch, int *status)
+                                           uint32_t *flag, uint32_t epoch, int *status, __bf16 *split = nullptr,
+                                           int64_t split_lo = 0)
 {
     const int tid = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -549,7 +562,8 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__
         }
         global_barrier();  // this role's updated rows are in L2 before other waves read them back
         if (!wait_panel_flag(s_ok, flag, epoch, status) || factor_failed(info)) return;
-        trsm_rows<false, T>(smem, Ck, lda, rows, (const T *)nullptr, 0, 128, nullptr, nullptr, Wp);
+        trsm_rows<false, T>(smem, Ck, lda, rows, (const T *)nullptr, 0, 128, nullptr, nullptr, Wp, split, split_lo,
+                            R0);
     }
 }
 

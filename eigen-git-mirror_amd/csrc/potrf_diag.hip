@@ -19,7 +19,7 @@
 
 namespace mi {
 
-// Workgroup b: X_b^T = L_bb^{-T} to Wout + b*kInvDoubles (ld 128) and X_b right after it.
+// Workgroup b: X_b^T = L_bb^{-T} to Wout + b*kInvDoubles (ld 128) and X_b at + kInvX.
 template <typename T>
 __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict__ L, int64_t ldl, int64_t n,
                                                             const double *__restrict__ Wleaf,
@@ -32,6 +32,64 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict_
     const int bs = (int)((n - kb) < 128 ? (n - kb) : 128);
     trsm_rows<true, T>(smem, (T *)nullptr, 0, 128, L + kb + kb * ldl, ldl, bs,
                        Wleaf + (size_t)b * kPanelDoubles + kLeafOffset, Wout + (size_t)b * kInvDoubles);
+}
+
+// Couplings of the block substitution: with them a block of the solution depends on its
+// predecessor through one product, Y_i = X_i (B_i - sum_{j<i-1} L_ij Y_j) - M_i Y_{i-1}, so
+// everything but that product is done before the predecessor is published.  Workgroup b <
+// nblk: M_b = X_b L_{b,b-1} (b >= 1); workgroup nblk + b: N_b = X_b^T L_{b+1,b}^T
+// (b <= nblk - 2).  128 x 128 each at Winv[b] + kInvM / kInvN (ld 128); entries of L outside
+// the matrix read as 0.  4 waves of 64 x 64 (4 x 4 FP64 MFMA tiles), depth 128.
+template <typename T>
+__global__ __launch_bounds__(256) void block_coupling_kernel(const T *__restrict__ L, int64_t ldl, int64_t n,
+                                                             double *__restrict__ Winv, const int *info)
+{
+    if (factor_failed(info)) return;
+    const int nblk = (int)((n + 127) / 128);
+    const bool fwd = (int)blockIdx.x < nblk;
+    const int b = fwd ? (int)blockIdx.x : (int)blockIdx.x - nblk;
+    if (fwd ? b < 1 : b > nblk - 2) return;
+    double *Wb = Winv + (size_t)b * kInvDoubles;
+    const double *P = Wb + (fwd ? kInvX : 0);  // P[i][k] = X_b[i][k] or X_b^T[i][k] at P + i + 128 k
+    // Q[j][k] = L[128 b + k][128 (b-1) + j] (forward; rows k < kv exist) or
+    //           L[128 (b+1) + j][128 b + k] (backward; rows j < jv exist)
+    const int64_t r0 = fwd ? 128 * (int64_t)b : 128 * (int64_t)(b + 1);
+    const int64_t c0 = fwd ? 128 * (int64_t)(b - 1) : 128 * (int64_t)b;
+    const int lim = (int)((n - r0) < 128 ? (n - r0) : 128);
+    const int l = lane_id(), w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1, kl = l >> 4;
+    dvec4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[a][q] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int s = 0; s < 32; ++s) {
+        const int k = 4 * s + kl;
+        double pv[4], qv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) pv[a] = P[64 * wr + 16 * a + (l & 15) + 128 * k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = 64 * wc + 16 * q + (l & 15);
+            const int rr = fwd ? k : j;  // the row of L read, relative to r0
+            const bool in = rr < lim;
+            const int64_t off = fwd ? (r0 + (in ? k : 0)) + (c0 + j) * ldl : (r0 + (in ? j : 0)) + (c0 + k) * ldl;
+            const double v = (double)L[off];
+            qv[q] = in ? v : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[a][q] = mfma_f64(qv[q], pv[a], acc[a][q]);
+    }
+    double *C = Wb + (fwd ? kInvM : kInvN);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                C[(64 * wr + 16 * a + (l & 15)) + 128 * (64 * wc + 16 * q + 4 * r + kl)] = acc[a][q][r];
 }
 
 // Leaf inverses of the diagonal blocks of an already factored matrix: workgroup b, wave w
@@ -85,6 +143,8 @@ void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wle
 {
     const unsigned nblk = (unsigned)((n + 127) / 128);
     hipLaunchKernelGGL(block_inverse_kernel<T>, dim3(nblk), dim3(256), 0, s, L, lda, n, Wleaf, Winv, info);
+    if (nblk > 1)
+        hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(2 * nblk), dim3(256), 0, s, L, lda, n, Winv, info);
 }
 
 template <typename T>

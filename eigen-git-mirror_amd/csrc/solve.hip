@@ -6,24 +6,27 @@
 // transposed view).  Blocked by the factorization's 128-row blocks:
 //   forward : Y_i = X_i (B_i - sum_{j<i} L_ij Y_j)
 //   backward: X_i = X_i^T (Y_i - sum_{j>i} L_ji^T X_j),      X_i = L_ii^{-1}
-// with the diagonal-block inverses built after the factorization.
+// with the diagonal-block inverses built after the factorization (and the couplings to the
+// predecessor block, which take X_i's product off the chain from block to block).
 //
 // Work items are (row block i, group of 16 right-hand sides).  A workgroup takes the next
 // item from an atomic ticket counter, in dependency order, so every block it waits for
-// belongs to a workgroup that started earlier and is resident: the waits always end.  Each
-// finished block is published with a release store of the call's epoch into its flag;
-// consumers poll with relaxed loads and issue one acquire fence once the flag is set.  Flags
-// carry an increasing epoch, so they never need clearing; only the ticket counter is reset
-// (by a one-thread kernel) before each sweep.
-// A poll that exceeds a large bound records MI_SOLVE_TIMEOUT in the status word and the
-// item is abandoned instead of spinning forever.
+// belongs to a workgroup that started earlier and is resident: the waits always end.
+//
+// A finished block is published as a packed copy [c + 16 * row] (the LDS layout of its
+// consumers, zero-padded to 128 x 16) written with agent-scope atomic stores, and the data
+// is its own flag: before each sweep the copies are filled with a signalling NaN, which no
+// arithmetic result can be (results of arithmetic are quiet NaNs), and a consumer polls one
+// element, then loads all 2048 and re-polls any still unwritten.  So a producer never
+// waits for its stores to drain and no separate flag is written or read.  A poll that
+// exceeds a large bound records MI_SOLVE_TIMEOUT in the status word and the item is
+// abandoned instead of spinning forever.
 //
 // Contributions are accumulated as soon as their block is published (left-looking per row
-// block): 8 waves, one 16-row MFMA tile each, K = 128 per contributing block, Y_j staged
-// through LDS (it is the shared operand of all 8 waves) as [rhs + 16 * row], so that the
-// 64 lanes of an MFMA operand read 64 consecutive doubles (no bank conflicts).  Barriers
-// that only protect LDS wait for LDS alone, so the prefetch of the next operand stays in
-// flight across them.
+// block): 8 waves, one 16-row MFMA tile each, K = 128 per contributing block, the block
+// staged through LDS (it is the shared operand of all 8 waves), so that the 64 lanes of an
+// MFMA operand read 64 consecutive doubles (no bank conflicts).  Barriers that only protect
+// LDS wait for LDS alone, so the prefetch of the next operand stays in flight across them.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -31,7 +34,10 @@ namespace mi {
 
 constexpr int SB = 128;          // row block (= factorization block)
 constexpr int SC = 16;           // right-hand sides per work item
+constexpr int SP = SB * SC;      // doubles per packed block
 constexpr uint32_t kSpinLimit = 1u << 22;  // ~0.2 s of polling
+// Unwritten marker of the packed copies: a signalling NaN (quiet bit 51 clear).
+constexpr unsigned long long kUnwritten = 0x7ff40000000005eeull;
 
 // Workgroup barrier ordering LDS accesses only (outstanding global loads stay in flight).
 __device__ __forceinline__ void lds_barrier()
@@ -41,31 +47,19 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Waits until *flag >= epoch (called by one thread; the caller's barrier then orders the
-// workgroup's reads after it).  Polls are relaxed agent-scope loads, which read the
-// coherent value without invalidating caches; a single acquire fence follows success.  (An
-// acquire on every poll invalidates the caches each time, and with hundreds of polling
-// workgroups that starves every other memory access.)  Gives up, recording
-// MI_SOLVE_TIMEOUT, after kSpinLimit polls, or at once when another workgroup gave up.
-__device__ __forceinline__ bool wait_flag(const uint32_t *flag, uint32_t epoch, int *status)
-{
-    uint32_t spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-        __builtin_amdgcn_s_sleep(2);
-        if ((++spins & 255) == 0 && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-            return false;
-        if (spins > kSpinLimit) {
-            atomicOr(status, MI_SOLVE_TIMEOUT);
-            return false;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return true;
-}
+__device__ __forceinline__ bool unwritten(double v) { return (unsigned long long)__double_as_longlong(v) == kUnwritten; }
 
 // BACKWARD = false: forward sweep with L; true: backward sweep with L^T.
-// Winv holds, per block b, X_b^T (ld 128) at Winv + b*2*16384 and X_b at +16384.
+// Winv holds, per block b (at Winv + b * kInvDoubles, each ld 128): X_b^T, X_b, and the
+// couplings M_b = X_b L_{b,b-1}, N_b = X_b^T L_{b+1,b}^T.  Ypk: per group of right-hand
+// sides, nblk packed blocks, filled with kUnwritten before the launch.
 // T: storage of L (float for a mixed-precision factor, widened on load; FP64 arithmetic).
+//
+// Block i depends on its predecessor p (i-1 forward, i+1 backward) through one product:
+//   forward : Y_i = Z_i - M_i Y_{i-1},  Z_i = X_i (B_i - sum_{j<i-1} L_ij Y_j)
+//   backward: X_i = Z_i - N_i X_{i+1},  Z_i = X_i^T (Y_i - sum_{j>i+1} L_ji^T X_j)
+// so a workgroup finishes Z_i while p is still being computed, and the step from one block
+// to the next is: see p, stage it, 32 MFMA steps (on four independent chains), store.
 //
 // Each wave owns one 16-row tile of the block, rt = w for waves 0..3 and 11 - w for waves
 // 4..7: waves w and w+4 share a SIMD, and the triangular X_i product costs row tile rt
@@ -73,20 +67,22 @@ __device__ __forceinline__ bool wait_flag(const uint32_t *flag, uint32_t epoch, 
 template <bool BACKWARD, typename T>
 __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ L, int64_t lda, int64_t n,
                                                           double *__restrict__ B, int64_t ldb, int64_t nrhs,
-                                                          const double *__restrict__ Winv, uint32_t *flags,
-                                                          int *ticket, uint32_t epoch, const int *info, int *status)
+                                                          const double *__restrict__ Winv, double *Ypk, int *ticket,
+                                                          const int *info, int *status)
 {
     if (factor_failed(info)) return;  // uniform: info is final before this kernel starts
-    __shared__ double Ys[SB * SC];    // Ys[c + SC*k]: the published block being consumed
-    __shared__ double Ts[SB * SC];    // Ts[c + SC*r]: the block's residual before the inverse
-    __shared__ int s_item;
-    __shared__ int s_ok;
+    __shared__ double Ys[SP];         // Ys[c + SC*k]: the published block being consumed
+    __shared__ double Ts[SP];         // Ts[c + SC*r]: the block's residual before the inverse
+    __shared__ int s_item, s_ok, s_fail;
     const int nblk = (int)((n + SB - 1) / SB);
     const int ngrp = (int)((nrhs + SC - 1) / SC);
     const int tid = threadIdx.x, l = lane_id();
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-    if (tid == 0) s_item = atomicAdd(ticket, 1);
+    if (tid == 0) {
+        s_item = atomicAdd(ticket, 1);
+        s_fail = 0;
+    }
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(s_item);
     if (t >= nblk * ngrp) return;
@@ -95,13 +91,14 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
     const int64_t i0 = (int64_t)ib * SB, c0 = (int64_t)g * SC;
     const int bi = (int)((n - i0) < SB ? (n - i0) : SB);
     const int nc = (int)((nrhs - c0) < SC ? (nrhs - c0) : SC);
-    uint32_t *gflags = flags + (size_t)g * nblk;
+    double *gpk = Ypk + (size_t)g * nblk * SP;  // block jb of this group at gpk + jb * SP
 
     // this wave's 16 rows of the block
     const int rt = w < 4 ? w : 11 - w;
     const int r = 16 * rt + (l & 15);
     const int rc = r < bi ? r : bi - 1;  // clamped row (in bounds)
     const int kl = l >> 4;
+    const double *Wi = Winv + (size_t)ib * kInvDoubles;
 
     // B_i (this wave's rows; lane l, element e: row r, column 4e + l/16): read before any wait
     double bv[4];
@@ -110,57 +107,111 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         const int c = 4 * e + kl;
         bv[e] = B[(i0 + rc) + (c0 + (c < nc ? c : 0)) * ldb];
     }
-
-    // P operand of a contributing block jb, all 32 k-steps of this lane at once:
-    // P[row][k] = L_ij[row][k] (forward) or L_ji[k][row] (backward); k clamped in bounds.
+    // the predecessor and the coupling to it (P[row][k] = M_i[row][k] or N_i[row][k])
     const int jstep = BACKWARD ? -1 : 1;
-    double pv[SB / 4];
-#define MI_LOAD_OPERAND(jb_)                                                                       \
-    do {                                                                                           \
-        const int64_t j0_ = (int64_t)(jb_)*SB;                                                     \
-        const int bj_ = (int)((n - j0_) < SB ? (n - j0_) : SB);                                    \
-        const T *colbase_ = BACKWARD ? L + j0_ + (i0 + rc) * lda : L + (i0 + rc) + j0_ * lda;       \
-        _Pragma("unroll") for (int s = 0; s < SB / 4; ++s)                                         \
-        {                                                                                          \
-            const int k_ = (4 * s + kl) < bj_ ? (4 * s + kl) : bj_ - 1;                             \
-            pv[s] = (double)colbase_[BACKWARD ? (int64_t)k_ : (int64_t)k_ * lda];                  \
-        }                                                                                          \
-    } while (0)
+    const bool crit = BACKWARD ? ib + 1 < nblk : ib > 0;
+    const int jend = crit ? ib - jstep : ib;  // contributions before the predecessor: [start, jend)
+    const double *Mp = Wi + (BACKWARD ? kInvN : kInvM) + r + SB * kl;
 
-    dvec4 acc = {0.0, 0.0, 0.0, 0.0};
-    int jb = BACKWARD ? nblk - 1 : 0;
-    if (jb != ib) MI_LOAD_OPERAND(jb);
-    for (; jb != ib; jb += jstep) {
-        if (tid == 0) s_ok = wait_flag(gflags + jb, epoch, status) ? 1 : 0;
-        __syncthreads();
-        if (!s_ok) return;
-        const int64_t j0 = (int64_t)jb * SB;
-        const int bj = (int)((n - j0) < SB ? (n - j0) : SB);
-        for (int e = tid; e < SB * SC; e += 512) {  // Y_j (bj x nc, zero-padded to 128 x 16)
-            const int c = e & (SC - 1), k = e >> 4;
-            const bool in = k < bj && c < nc;
-            const double v = coherent_load(B + (j0 + (in ? k : 0)) + (c0 + (in ? c : 0)) * ldb);
-            Ys[e] = in ? v : 0.0;
+    // Stages packed block jb into Ys once it is written (false: given up).  Thread 0 polls
+    // the last element; then every thread loads its four and re-polls any not yet visible.
+    auto stage = [&](int jb) -> bool {
+        const double *src = gpk + (size_t)jb * SP;
+        if (tid == 0) {
+            uint32_t spins = 0;
+            int ok = 1;
+            while (unwritten(coherent_load(src + SP - 1))) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((++spins & 255) == 0 && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ok = 0;
+                    break;
+                }
+                if (spins > kSpinLimit) {
+                    atomicOr(status, MI_SOLVE_TIMEOUT);
+                    ok = 0;
+                    break;
+                }
+            }
+            s_ok = ok;
         }
         lds_barrier();
-        // acc[row][c] += sum_k P[row][k] * Ys[k][c]; rows k >= bj of Ys are zero, so the
-        // clamped (finite) P values past a partial block contribute nothing
+        if (!s_ok) return false;
+        double v[SP / 512];
 #pragma unroll
-        for (int s = 0; s < SB / 4; ++s) acc = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], acc);
-        if (jb + jstep != ib) MI_LOAD_OPERAND(jb + jstep);  // next block: in flight during the next wait
-        lds_barrier();  // Ys is rewritten by the next contribution
-    }
-#undef MI_LOAD_OPERAND
+        for (int e = 0; e < SP / 512; ++e) v[e] = coherent_load(src + e * 512 + tid);
+#pragma unroll
+        for (int e = 0; e < SP / 512; ++e) {
+            uint32_t spins = 0;
+            while (unwritten(v[e])) {
+                __builtin_amdgcn_s_sleep(1);
+                v[e] = coherent_load(src + e * 512 + tid);
+                if (++spins > kSpinLimit) {
+                    atomicOr(status, MI_SOLVE_TIMEOUT);
+                    s_fail = 1;
+                    break;
+                }
+            }
+            Ys[e * 512 + tid] = v[e];
+        }
+        lds_barrier();
+        return s_fail == 0;
+    };
 
+    // P operand of a contributing block, k-step s: P[row][k] = L_ij[row][k] (forward) or
+    // L_ji[k][row] (backward); k clamped in bounds.  `aim` points it at block jb.
+    const T *lb = nullptr;  // this lane's row (forward) or column (backward) of the block
+    int bjm = SB - 1;       // last valid k of the block
+    auto aim = [&](int jb) {
+        const int64_t j0 = (int64_t)jb * SB;
+        bjm = (int)((n - j0) < SB ? (n - j0) : SB) - 1;
+        lb = BACKWARD ? L + j0 + (i0 + rc) * lda : L + (i0 + rc) + j0 * lda;
+    };
+    auto operand = [&](int s) -> double {
+        const int k = (4 * s + kl) < bjm ? (4 * s + kl) : bjm;
+        return (double)lb[BACKWARD ? (int64_t)k : (int64_t)k * lda];
+    };
     // op(X_i): column-major P[row][k] = X_i[row][k] (forward, lower: k < 16(rt+1)) or X_i[k][row]
     // (backward, upper: k >= 16 rt); only the nonzero k-steps are loaded and multiplied
-    const double *Xp = Winv + (size_t)ib * 2 * SB * SB + (BACKWARD ? 0 : SB * SB) + r + SB * kl;
+    const double *Xp = Wi + (BACKWARD ? 0 : kInvX) + r + SB * kl;
     const int s_lo = BACKWARD ? 4 * rt : 0;
     const int s_hi = BACKWARD ? SB / 4 : 4 * (rt + 1);
-#pragma unroll
-    for (int s = 0; s < SB / 4; ++s)
-        if (s >= s_lo && s < s_hi) pv[s] = Xp[(size_t)4 * SB * s];
 
+    double pv[SB / 4];
+    dvec4 acc = {0.0, 0.0, 0.0, 0.0};
+    int jb = BACKWARD ? nblk - 1 : 0;
+    const bool contrib = jb != jend;  // any contributions before the predecessor's
+    if (contrib) {
+        aim(jb);
+#pragma unroll
+        for (int s = 0; s < SB / 4; ++s) pv[s] = operand(s);
+    }
+    for (; jb != jend; jb += jstep) {
+        if (!stage(jb)) return;
+        // acc[row][c] += sum_k P[row][k] * Ys[k][c]; rows k >= bj of Ys are zero, so the
+        // clamped (finite) P values past a partial block contribute nothing.  In the last
+        // contribution each k-step's operand register is refilled with X_i's right after its
+        // MFMA issues, so X_i is in flight during these MFMAs.
+        if (jb + jstep != jend) {
+#pragma unroll
+            for (int s = 0; s < SB / 4; ++s) acc = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], acc);
+            aim(jb + jstep);
+#pragma unroll
+            for (int s = 0; s < SB / 4; ++s) pv[s] = operand(s);  // in flight during the next wait
+        } else {
+#pragma unroll
+            for (int s = 0; s < SB / 4; ++s) {
+                acc = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], acc);
+                if (s >= s_lo && s < s_hi) pv[s] = Xp[(size_t)4 * SB * s];
+            }
+        }
+        lds_barrier();  // Ys is rewritten by the next contribution
+    }
+
+    if (!contrib) {
+#pragma unroll
+        for (int s = 0; s < SB / 4; ++s)
+            if (s >= s_lo && s < s_hi) pv[s] = Xp[(size_t)4 * SB * s];
+    }
     // T = B_i - acc  (lane l, element e: row r, column 4e + l/16)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -169,50 +220,79 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         Ts[c + SC * r] = in ? bv[e] - acc[e] : 0.0;
     }
     lds_barrier();
-    // Y_i = op(X_i) T
+    // Z_i = op(X_i) T; each k-step's operand register then takes the coupling's, which is in
+    // flight while the predecessor is awaited and staged
     dvec4 y = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < SB / 4; ++s)
+    for (int s = 0; s < SB / 4; ++s) {
         if (s >= s_lo && s < s_hi) y = mfma_f64(Ts[(l & 15) + SC * (4 * s + kl)], pv[s], y);
+        if (crit) pv[s] = Mp[(size_t)4 * SB * s];
+    }
+
+    // the step that waits: Y_i = Z_i - (M_i or N_i) Y_p, on four independent MFMA chains
+    if (crit) {
+        if (!stage(jend)) return;
+        dvec4 cq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cq[q] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < SB / 4; ++s) cq[s & 3] = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], cq[s & 3]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) y[e] -= (cq[0][e] + cq[1][e]) + (cq[2][e] + cq[3][e]);
+    }
+    // publish: the packed copy (atomic stores, so they reach every XCD without a flush), then
+    // the result itself
+    double *dst = gpk + (size_t)ib * SP;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int c = 4 * e + kl;
+        const bool in = r < bi && c < nc;
+        double v = in ? y[e] : 0.0;
+        if (unwritten(v)) v = __builtin_nan("");  // never: arithmetic yields quiet NaNs
+        __hip_atomic_store(dst + c + SC * r, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int c = 4 * e + kl;
         if (r < bi && c < nc) B[(i0 + r) + (c0 + c) * ldb] = y[e];
     }
-    // every wave's stores reach L2 before the barrier; one agent-scope release store then
-    // publishes them all
-    global_barrier();
-    if (tid == 0) __hip_atomic_store(gflags + ib, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ void reset_ticket_kernel(int *ticket) { *ticket = 0; }
+// Before a sweep: the ticket counter to 0 and the sweep's packed copies to kUnwritten.
+__global__ __launch_bounds__(256) void solve_reset_kernel(int *ticket, double *Ypk, int64_t count)
+{
+    const int64_t i0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i0 == 0) *ticket = 0;
+    const double u = __longlong_as_double((long long)kUnwritten);
+    for (int64_t i = i0; i < count; i += (int64_t)gridDim.x * 256) Ypk[i] = u;
+}
 
-// flags: ngrp * nblk uint32 (persistent across calls, epochs increase); tickets: 2 ints.
+// tickets: 2 ints; Ypk: 2 * solve_pack_doubles(n, nrhs) doubles (forward, backward copies).
 template <typename T>
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
-                             const double *Winv, uint32_t *flags, int *tickets, uint32_t epoch_fwd,
-                             uint32_t epoch_bwd, const int *info, int *status, hipStream_t s, int sweeps)
+                             const double *Winv, double *Ypk, int *tickets, const int *info, int *status, hipStream_t s,
+                             int sweeps)
 {
     const int nblk = (int)((n + SB - 1) / SB);
     const int ngrp = (int)((nrhs + SC - 1) / SC);
     const unsigned items = (unsigned)(nblk * ngrp);
+    const int64_t pk = solve_pack_doubles(n, nrhs);
+    const unsigned fill = (unsigned)((pk + 255) / 256 < 1024 ? (pk + 255) / 256 : 1024);
     if (sweeps & 1) {
-        hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, s, tickets);
+        hipLaunchKernelGGL(solve_reset_kernel, dim3(fill), dim3(256), 0, s, tickets, Ypk, pk);
         hipLaunchKernelGGL((solve_sweep_kernel<false, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv,
-                           flags, tickets, epoch_fwd, info, status);
+                           Ypk, tickets, info, status);
     }
     if (sweeps & 2) {
-        hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, s, tickets + 1);
+        hipLaunchKernelGGL(solve_reset_kernel, dim3(fill), dim3(256), 0, s, tickets + 1, Ypk + pk, pk);
         hipLaunchKernelGGL((solve_sweep_kernel<true, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv,
-                           flags, tickets + 1, epoch_bwd, info, status);
+                           Ypk + pk, tickets + 1, info, status);
     }
 }
 
 template void launch_solve_persistent<double>(const double *, int64_t, int64_t, double *, int64_t, int64_t,
-                                              const double *, uint32_t *, int *, uint32_t, uint32_t, const int *,
-                                              int *, hipStream_t, int);
+                                              const double *, double *, int *, const int *, int *, hipStream_t, int);
 template void launch_solve_persistent<float>(const float *, int64_t, int64_t, double *, int64_t, int64_t,
-                                             const double *, uint32_t *, int *, uint32_t, uint32_t, const int *,
-                                             int *, hipStream_t, int);
+                                             const double *, double *, int *, const int *, int *, hipStream_t, int);
 
 }  // namespace mi

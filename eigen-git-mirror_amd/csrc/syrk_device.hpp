@@ -18,14 +18,42 @@ template <int K, typename T = double, int PREC = kPrecF64>
 __device__ __forceinline__ void syrk_tile(T *__restrict__ C, int64_t ldc, const T *__restrict__ P, int64_t ldp,
                                           int m, int ncol, int ti, int tj)
 {
-    using Tile = typename TileSel<T, PREC>::type;
     const int w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
     if (ti == tj && wr == 0 && wc == 1) return;
     const int i0 = ti * 128 + wr * 64, j0 = tj * 128 + wc * 64;
-    Tile t;
+    if constexpr (sizeof(T) == 8) {
+        // C is read into the accumulators first, so its loads overlap the first operand
+        // prefetch, and the negated product accumulates into it: the tile ends with stores only
+        // (about 2% off the factorization at n = 8192 against a read of C after the last MFMA)
+        WaveTile<4, 4, 3, true> t;
+        const int rows[4] = {i0, i0 + 16, i0 + 32, i0 + 48};
+        t.load_acc(C, ldc, rows, j0, m, ncol);
+        t.setup(P, ldp, i0, m, P, ldp, j0, m, K);
+        t.template run_fixed<K / 4>();
+        t.template store<true, true>(C, ldc, i0, j0, m, ncol);
+    } else {
+        using Tile = typename TileSel<T, PREC>::type;
+        Tile t;
+        t.zero();
+        t.setup(P, ldp, i0, m, P, ldp, j0, m, K);
+        t.template run_fixed<K / Tile::KS>();
+        t.template store<false, true>(C, ldc, i0, j0, m, ncol);
+    }
+}
+
+// The split-BF16 form (FP32 storage): the K-deep operand comes from split-panel slots, S0 for
+// depth columns 0..127 and S1 for 128..255; C's row 0 is row r0 of the slots.
+template <int K>
+__device__ __forceinline__ void syrk_tile_split(float *__restrict__ C, int64_t ldc, const __bf16 *S0, const __bf16 *S1,
+                                                int64_t lo_off, int r0, int m, int ncol, int ti, int tj)
+{
+    const int w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    if (ti == tj && wr == 0 && wc == 1) return;
+    const int i0 = ti * 128 + wr * 64, j0 = tj * 128 + wc * 64;
+    WaveTileSplit<4, 4, 1> t;
     t.zero();
-    t.setup(P, ldp, i0, m, P, ldp, j0, m, K);
-    t.template run_fixed<K / Tile::KS>();
+    t.setup(S0, S1, lo_off, r0, i0, m, j0, m);
+    t.template run_fixed<K / 32>();
     t.template store<false, true>(C, ldc, i0, j0, m, ncol);
 }
 

@@ -41,6 +41,10 @@ def test_mixed_reaches_fp64_backward_error(dev, prec, n, kappa):
     if prec == "fp32":
         # kappa * u_fp32 <= 0.06: classical refinement converges without the FP64 fallback
         assert iters >= 1
+    if prec == "bf16x3" and kappa <= 1e4:
+        # split BF16 (~16 significant bits): a few corrections, no fallback -- a factor from
+        # wrongly split operands would not converge this fast
+        assert 1 <= iters <= 3, iters
     assert iters >= -2
 
 
