@@ -18,10 +18,12 @@ constexpr int kNB = 128;
 //   Winv[b] : L_bb^{-T}, 128 x 128, ld 128 (built for the solve)
 constexpr int64_t kPanelDoubles = 36 * 256;
 constexpr int64_t kLeafOffset = 28 * 256;
-// Per diagonal block b, four 128 x 128 matrices (ld 128): X_b^T, X_b (X_b = L_bb^{-1}), then the
-// couplings of the solve M_b = X_b L_{b,b-1} and N_b = X_b^T L_{b+1,b}^T (zero where undefined).
-constexpr int64_t kInvDoubles = 4 * 128 * 128;
+// Per diagonal block b, six 128 x 128 matrices (ld 128): X_b^T, X_b (X_b = L_bb^{-1}), then the
+// couplings of the solve M_b = X_b L_{b,b-1}, N_b = X_b^T L_{b+1,b}^T, M2_b = X_b L_{b,b-2},
+// N2_b = X_b^T L_{b+2,b}^T (zero where undefined).
+constexpr int64_t kInvDoubles = 6 * 128 * 128;
 constexpr int64_t kInvX = 128 * 128, kInvM = 2 * 128 * 128, kInvN = 3 * 128 * 128;  // offsets
+constexpr int64_t kInvM2 = 4 * 128 * 128, kInvN2 = 5 * 128 * 128;
 
 // Device status bits (a word in the handle's persistent device memory).
 constexpr int MI_SOLVE_TIMEOUT = 1;
@@ -42,7 +44,7 @@ void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, dou
 // BF16 elements of one slot of the split-panel ring (Wsplit, 4 slots) of the split-BF16
 // factorization: the hi parts of n rows (rounded up to 128) x 128 columns, then the lo parts.
 inline int64_t split_slot_elems(int64_t n) { return 2 * ((n + 127) / 128 * 128) * 128; }
-// Winv[b] = {X_b^T, X_b, M_b, N_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).
+// Winv[b] = {X_b^T, X_b, M_b, N_b, M2_b, N2_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).
 template <typename T>
 void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv, const int *info,
                            hipStream_t s);

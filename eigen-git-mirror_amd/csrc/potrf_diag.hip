@@ -34,27 +34,30 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict_
                        Wleaf + (size_t)b * kPanelDoubles + kLeafOffset, Wout + (size_t)b * kInvDoubles);
 }
 
-// Couplings of the block substitution: with them a block of the solution depends on its
-// predecessor through one product, Y_i = X_i (B_i - sum_{j<i-1} L_ij Y_j) - M_i Y_{i-1}, so
-// everything but that product is done before the predecessor is published.  Workgroup b <
-// nblk: M_b = X_b L_{b,b-1} (b >= 1); workgroup nblk + b: N_b = X_b^T L_{b+1,b}^T
-// (b <= nblk - 2).  128 x 128 each at Winv[b] + kInvM / kInvN (ld 128); entries of L outside
-// the matrix read as 0.  4 waves of 64 x 64 (4 x 4 FP64 MFMA tiles), depth 128.
+// Couplings of the block substitution: with them a block of the solution depends on its two
+// predecessors through one product each,
+//   Y_i = X_i (B_i - sum_{j<i-2} L_ij Y_j) - M2_i Y_{i-2} - M_i Y_{i-1},
+// so everything else is done before they are published.  Workgroup q nblk + b, distance
+// d = 1 + q / 2: q even, M_b / M2_b = X_b L_{b,b-d} (b >= d); q odd, N_b / N2_b =
+// X_b^T L_{b+d,b}^T (b + d < nblk).  128 x 128 each at Winv[b] + kInvM / kInvN / kInvM2 /
+// kInvN2 (ld 128); entries of L outside the matrix read as 0.  4 waves of 64 x 64 (4 x 4 FP64
+// MFMA tiles), depth 128.
 template <typename T>
 __global__ __launch_bounds__(256) void block_coupling_kernel(const T *__restrict__ L, int64_t ldl, int64_t n,
                                                              double *__restrict__ Winv, const int *info)
 {
     if (factor_failed(info)) return;
     const int nblk = (int)((n + 127) / 128);
-    const bool fwd = (int)blockIdx.x < nblk;
-    const int b = fwd ? (int)blockIdx.x : (int)blockIdx.x - nblk;
-    if (fwd ? b < 1 : b > nblk - 2) return;
+    const int q = (int)blockIdx.x / nblk, b = (int)blockIdx.x % nblk;
+    const bool fwd = (q & 1) == 0;
+    const int d = 1 + q / 2;
+    if (fwd ? b < d : b + d > nblk - 1) return;
     double *Wb = Winv + (size_t)b * kInvDoubles;
     const double *P = Wb + (fwd ? kInvX : 0);  // P[i][k] = X_b[i][k] or X_b^T[i][k] at P + i + 128 k
-    // Q[j][k] = L[128 b + k][128 (b-1) + j] (forward; rows k < kv exist) or
-    //           L[128 (b+1) + j][128 b + k] (backward; rows j < jv exist)
-    const int64_t r0 = fwd ? 128 * (int64_t)b : 128 * (int64_t)(b + 1);
-    const int64_t c0 = fwd ? 128 * (int64_t)(b - 1) : 128 * (int64_t)b;
+    // Q[j][k] = L[128 b + k][128 (b-d) + j] (forward; rows k < lim exist) or
+    //           L[128 (b+d) + j][128 b + k] (backward; rows j < lim exist)
+    const int64_t r0 = fwd ? 128 * (int64_t)b : 128 * (int64_t)(b + d);
+    const int64_t c0 = fwd ? 128 * (int64_t)(b - d) : 128 * (int64_t)b;
     const int lim = (int)((n - r0) < 128 ? (n - r0) : 128);
     const int l = lane_id(), w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1, kl = l >> 4;
     dvec4 acc[4][4];
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(256) void block_coupling_kernel(const T *__restrict
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc[a][q] = mfma_f64(qv[q], pv[a], acc[a][q]);
     }
-    double *C = Wb + (fwd ? kInvM : kInvN);
+    double *C = Wb + (fwd ? (d == 1 ? kInvM : kInvM2) : (d == 1 ? kInvN : kInvN2));
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -144,7 +147,7 @@ void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wle
     const unsigned nblk = (unsigned)((n + 127) / 128);
     hipLaunchKernelGGL(block_inverse_kernel<T>, dim3(nblk), dim3(256), 0, s, L, lda, n, Wleaf, Winv, info);
     if (nblk > 1)
-        hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(2 * nblk), dim3(256), 0, s, L, lda, n, Winv, info);
+        hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(4 * nblk), dim3(256), 0, s, L, lda, n, Winv, info);
 }
 
 template <typename T>

@@ -55,11 +55,13 @@ __device__ __forceinline__ bool unwritten(double v) { return (unsigned long long
 // sides, nblk packed blocks, filled with kUnwritten before the launch.
 // T: storage of L (float for a mixed-precision factor, widened on load; FP64 arithmetic).
 //
-// Block i depends on its predecessor p (i-1 forward, i+1 backward) through one product:
-//   forward : Y_i = Z_i - M_i Y_{i-1},  Z_i = X_i (B_i - sum_{j<i-1} L_ij Y_j)
-//   backward: X_i = Z_i - N_i X_{i+1},  Z_i = X_i^T (Y_i - sum_{j>i+1} L_ji^T X_j)
-// so a workgroup finishes Z_i while p is still being computed, and the step from one block
-// to the next is: see p, stage it, 32 MFMA steps (on four independent chains), store.
+// Block i depends on its two predecessors (i-1, i-2 forward; i+1, i+2 backward) through one
+// product each:
+//   forward : Y_i = Z_i - M2_i Y_{i-2} - M_i Y_{i-1},  Z_i = X_i (B_i - sum_{j<i-2} L_ij Y_j)
+//   backward: X_i = Z_i - N2_i X_{i+2} - N_i X_{i+1},  Z_i = X_i^T (Y_i - sum_{j>i+2} L_ji^T X_j)
+// so a workgroup finishes Z_i (its triangular X_i product included) two blocks ahead, the
+// i-2 product one block ahead, and the step from one block to the next is: see i-1, stage
+// it, 32 MFMA steps (on four independent chains), store.
 //
 // Each wave owns one 16-row tile of the block, rt = w for waves 0..3 and 11 - w for waves
 // 4..7: waves w and w+4 share a SIMD, and the triangular X_i product costs row tile rt
@@ -107,11 +109,13 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         const int c = 4 * e + kl;
         bv[e] = B[(i0 + rc) + (c0 + (c < nc ? c : 0)) * ldb];
     }
-    // the predecessor and the coupling to it (P[row][k] = M_i[row][k] or N_i[row][k])
+    // the predecessors and the couplings to them (P[row][k] = M_i[row][k], M2_i[row][k], ...)
     const int jstep = BACKWARD ? -1 : 1;
-    const bool crit = BACKWARD ? ib + 1 < nblk : ib > 0;
-    const int jend = crit ? ib - jstep : ib;  // contributions before the predecessor: [start, jend)
-    const double *Mp = Wi + (BACKWARD ? kInvN : kInvM) + r + SB * kl;
+    const int npred = BACKWARD ? nblk - 1 - ib : ib;  // blocks before this one in the sweep
+    const bool crit1 = npred >= 1, crit2 = npred >= 2;
+    const int jend = ib - jstep * (npred < 2 ? npred : 2);  // plain contributions: [start, jend)
+    const double *M1p = Wi + (BACKWARD ? kInvN : kInvM) + r + SB * kl;
+    const double *M2p = Wi + (BACKWARD ? kInvN2 : kInvM2) + r + SB * kl;
 
     // Stages packed block jb into Ys once it is written (false: given up).  Thread 0 polls
     // the last element; then every thread loads its four and re-polls any not yet visible.
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
     double pv[SB / 4];
     dvec4 acc = {0.0, 0.0, 0.0, 0.0};
     int jb = BACKWARD ? nblk - 1 : 0;
-    const bool contrib = jb != jend;  // any contributions before the predecessor's
+    const bool contrib = jb != jend;  // any plain contributions (before the two coupled blocks)
     if (contrib) {
         aim(jb);
 #pragma unroll
@@ -220,19 +224,33 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         Ts[c + SC * r] = in ? bv[e] - acc[e] : 0.0;
     }
     lds_barrier();
-    // Z_i = op(X_i) T; each k-step's operand register then takes the coupling's, which is in
-    // flight while the predecessor is awaited and staged
+    // Z_i = op(X_i) T; each k-step's operand register then takes the first coupling's, which is
+    // in flight while that predecessor is awaited and staged
+    const double *Cp = crit2 ? M2p : M1p;
     dvec4 y = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < SB / 4; ++s) {
         if (s >= s_lo && s < s_hi) y = mfma_f64(Ts[(l & 15) + SC * (4 * s + kl)], pv[s], y);
-        if (crit) pv[s] = Mp[(size_t)4 * SB * s];
+        if (crit1) pv[s] = Cp[(size_t)4 * SB * s];
     }
 
-    // the step that waits: Y_i = Z_i - (M_i or N_i) Y_p, on four independent MFMA chains
-    if (crit) {
-        if (!stage(jend)) return;
-        dvec4 cq[4];
+    // the coupling products, y -= (M2_i or N2_i) Y_{i-2}, then y -= (M_i or N_i) Y_{i-1}, each on
+    // four independent MFMA chains; the second's operand loads follow the first's MFMAs
+    dvec4 cq[4];
+    if (crit2) {
+        if (!stage(ib - 2 * jstep)) return;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cq[q] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < SB / 4; ++s) {
+            cq[s & 3] = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], cq[s & 3]);
+            pv[s] = M1p[(size_t)4 * SB * s];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) y[e] -= (cq[0][e] + cq[1][e]) + (cq[2][e] + cq[3][e]);
+    }
+    if (crit1) {  // the step that waits
+        if (!stage(ib - jstep)) return;
 #pragma unroll
         for (int q = 0; q < 4; ++q) cq[q] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
