@@ -101,12 +101,20 @@ __global__ __launch_bounds__(256) void symm_part_kernel(const double *__restrict
         bool ok[U];
         if (transposed) {  // uniform over the workgroup
             // Ts[kk + 33 * ii] = A_sym(r0 + ii, k0 + kk) = A(k0 + kk, r0 + ii), stored (for either
-            // triangle) at A[k + r * lda]: lanes over kk, coalesced
-            for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-                const int kk = e & 31, ii = e >> 5;
+            // triangle) at A[k + r * lda]: lanes over kk, coalesced; all eight loads of a thread
+            // are issued before its first LDS store (one memory round trip, not eight)
+            double tv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = threadIdx.x + 256 * q, kk = e & 31, ii = e >> 5;
                 const int64_t k = k0 + kk, r = r0 + ii;
                 const int64_t kc2 = k < ke ? k : ke - 1, rc2 = r < n ? r : n - 1;
-                Ts[kk + 33 * ii] = A[kc2 + rc2 * lda];
+                tv[q] = A[kc2 + rc2 * lda];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = threadIdx.x + 256 * q;
+                Ts[(e & 31) + 33 * (e >> 5)] = tv[q];
             }
             __syncthreads();
 #pragma unroll
