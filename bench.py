@@ -126,7 +126,7 @@ def leg_mixed(n=16384, r=16, kappas=(1e2, 1e4, 1e6), reps=3):
         del da, asym
         case = {"kappa": kappa, "fp64": {"ms": round(t64, 3), "tflops": round(fl / t64 / 1e9, 3), "berr": be64}}
         best = None
-        for prec in ("fp32", "bf16x3", "bf16"):
+        for prec in ("fp32", "bf16x6", "bf16x3", "bf16"):
             res = {}
 
             def mixed():

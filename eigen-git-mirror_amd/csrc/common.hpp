@@ -6,9 +6,10 @@
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16 raw bytes
 
 // Arithmetic of the trailing updates (the values of mi_prec in mi_llt.h).
-enum : int { kPrecF64 = 0, kPrecF32 = 1, kPrecBF16 = 2, kPrecBF16x3 = 3 };
+enum : int { kPrecF64 = 0, kPrecF32 = 1, kPrecBF16 = 2, kPrecBF16x3 = 3, kPrecBF16x6 = 4 };
 
 // FP64 MFMA 16x16x4.  Lane layout of v_mfma_f64_16x16x4_f64, measured on gfx950:
 //   operand a: lane l supplies a[i = l%16][k = l/16]
@@ -63,6 +64,14 @@ template <typename T>
 __device__ __forceinline__ T coherent_load(const T *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroup barrier ordering LDS accesses only (outstanding global loads stay in flight).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Early-exit test used by every kernel of a factorization: a previous kernel on the

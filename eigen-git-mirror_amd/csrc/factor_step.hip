@@ -43,8 +43,8 @@ __device__ __forceinline__ void pair_tile(int u, int c0, int nblk, int &i, int &
 // S: storage type (double, or float for the mixed-precision factorization), PREC: the
 // trailing-update arithmetic (kPrecF64 with double; FP32/BF16/split BF16 MFMA with float).
 // Wsplit (split BF16 arithmetic only, else null): a ring of 4 panel slots of slot_elems BF16
-// each (hi parts of n rows x 128, then the lo parts); the row roles of panel q write slot q % 4
-// and the trailing update reads its depth from the slots of the slab's panels.
+// each (2 or 3 parts of split_part_elems(n): hi, lo or hi, mid, lo); the row roles of panel q
+// write slot q % 4 and the trailing update reads its depth from the slots of the slab's panels.
 // Workgroups per CU the kernel is compiled for: 2 fits two 72 KiB LDS blocks and 256 VGPRs
 // per lane on a CU (the second workgroup hides the first one's barrier and memory waits).
 #ifndef MI_FACTOR_WG_PER_CU
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
     if (Gp > 0 && (t == 0 || t > nsyrk)) {  // ticket 0: diagonal role; last Gp-1: row roles
         __bf16 *split = Wsplit ? Wsplit + ((kp / kNB) & 3) * slot_elems : nullptr;
         panel_role<S, PREC>(t == 0 ? 0 : t - nsyrk, smem, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch,
-                            status, split, slot_elems / 2);
+                            status, split, split_part_elems(n));
         return;
     }
     const int u = Gp > 0 ? t - 1 : t;
@@ -77,13 +77,14 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
     pair_tile(xcd_remap(u, nsyrk), c0, nblk, i, c);
     const int64_t r = (int64_t)c * kNB;
     const int64_t m = n - r;
-    if constexpr (sizeof(S) == 4 && PREC == kPrecBF16x3) {
+    if constexpr (sizeof(S) == 4 && (PREC == kPrecBF16x3 || PREC == kPrecBF16x6)) {
         if (Wsplit) {
             const int64_t sp = slab / kNB;  // first panel of the slab
             const __bf16 *S0 = Wsplit + (sp & 3) * slot_elems;
             const __bf16 *S1 = Wsplit + ((sp + 1) & 3) * slot_elems;
-            syrk_tile_split<K>(A + r + r * lda, lda, S0, S1, slot_elems / 2, (int)r, (int)m, (int)(m < kNB ? m : kNB),
-                               i - c, 0);
+            syrk_tile_split_lds<K, PREC == kPrecBF16x3 ? 2 : 3>(A + r + r * lda, lda, S0, S1, split_part_elems(n),
+                                                                (int)r, (int)m, (int)(m < kNB ? m : kNB), i - c,
+                                                                0, (char *)smem);
             return;
         }
     }
@@ -130,13 +131,16 @@ void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, dou
     *grid_out = grid;
     if (grid == 0) return;
     const dim3 g(grid);
-    const int64_t slot = split_slot_elems(n);
+    const int64_t slot = split_slot_elems(n, prec == kPrecBF16x6 ? 3 : 2);
     if constexpr (sizeof(S) == 8) {
         launch_prec<S, kPrecF64>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base, flag,
                                  epoch, status, nullptr, slot);
     } else if (prec == kPrecBF16) {
         launch_prec<S, kPrecBF16>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
                                   flag, epoch, status, nullptr, slot);
+    } else if (prec == kPrecBF16x6) {
+        launch_prec<S, kPrecBF16x6>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
+                                    flag, epoch, status, Wsplit, slot);
     } else if (prec == kPrecBF16x3) {
         launch_prec<S, kPrecBF16x3>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
                                     flag, epoch, status, Wsplit, slot);

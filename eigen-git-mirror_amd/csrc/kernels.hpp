@@ -35,15 +35,17 @@ constexpr int MI_PANEL_TIMEOUT = 2;
 // ticket_base = number of tickets drawn before this launch, epoch = this step's flag value.
 // S = double: FP64 throughout (prec ignored); S = float: FP32 storage, FP64 diagonal factor
 // and panel solve, trailing updates in the MFMA arithmetic `prec` (kPrecF32/BF16/BF16x3);
-// with kPrecBF16x3 and Wsplit given, the trailing update reads panels split once into BF16
-// hi/lo parts by the panel solve (4 * split_slot_elems(n) elements, no other contents needed).
+// with kPrecBF16x3 / kPrecBF16x6 and Wsplit given, the trailing update reads panels split once
+// into 2 / 3 BF16 parts by the panel solve (4 * split_slot_elems(n, parts) elements, no other
+// contents needed); without Wsplit, kPrecBF16x6 runs its trailing update on FP32 MFMAs.
 template <typename S>
 void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
                         hipStream_t s, uint32_t *grid_out, int prec, __bf16 *Wsplit = nullptr);
-// BF16 elements of one slot of the split-panel ring (Wsplit, 4 slots) of the split-BF16
-// factorization: the hi parts of n rows (rounded up to 128) x 128 columns, then the lo parts.
-inline int64_t split_slot_elems(int64_t n) { return 2 * ((n + 127) / 128 * 128) * 128; }
+// BF16 elements of one part (n rows rounded up to 128, x 128 columns) and of one slot (2 parts
+// for kPrecBF16x3, 3 for kPrecBF16x6) of the split-panel ring (Wsplit, 4 slots).
+__host__ __device__ inline int64_t split_part_elems(int64_t n) { return ((n + 127) / 128 * 128) * 128; }
+__host__ __device__ inline int64_t split_slot_elems(int64_t n, int parts) { return parts * split_part_elems(n); }
 // Winv[b] = {X_b^T, X_b, M_b, N_b, M2_b, N2_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).
 template <typename T>
 void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv, const int *info,

@@ -366,119 +366,37 @@ struct WaveTileF {
     __device__ __forceinline__ static int col_of(int b, int r) { return 16 * b + 4 * (lane_id() >> 4) + r; }
 };
 
-// Split-BF16 product from operands split once, when their panel was finished.  A panel slot
-// holds the hi parts (bf16(x)) and, lo_off elements later, the lo parts (bf16(x - hi)) of
-// every row in the MFMA operand order: per group of 16 rows, per k-step s (32 columns), per
-// lane quad q, per row, 8 values -- element (16 g + rr, s, q, e) at 2048 g + 512 s + 128 q +
-// 8 rr + e -- so the 16-byte operand loads of a wave's 16-row tile read 1 KiB contiguously.
-// The columns are permuted by the writer (split_store_row), the same way for every row, so a
-// product over k is unchanged.  Accumulator layout and store are WaveTileF's.
-template <int TA, int TB, int D = 1>
-struct WaveTileSplit : WaveTileF<TA, TB, kPrecBF16x3> {
-    static constexpr int U = D + 1;
-    const __bf16 *hp[2];      // per panel of the depth: its slot (hi part, row 0 of the matrix)
-    uint32_t po[TA], qo[TB];  // per-lane byte offsets of the operand rows
-    uint32_t lo_bytes;        // byte distance of the lo parts from the hi parts
-
-    __device__ __forceinline__ static uint32_t row_offset(int r)
-    {
-        return (uint32_t)(r >> 4) * 4096u + 256u * (uint32_t)(lane_id() >> 4) + 16u * (uint32_t)(r & 15);
-    }
-
-    // P rows r0 + p0 + .. (valid < r0 + pmax) and Q rows r0 + q0 + .. (valid < r0 + qmax) of
-    // the slots S0 (k-steps 0..3) and S1 (4..7).
-    __device__ __forceinline__ void setup(const __bf16 *S0, const __bf16 *S1, int64_t lo_off, int r0, int p0, int pmax,
-                                          int q0, int qmax)
-    {
-        const int l = lane_id();
-        hp[0] = S0;
-        hp[1] = S1;
-        lo_bytes = (uint32_t)(lo_off * 2);
-#pragma unroll
-        for (int a = 0; a < TA; ++a) {
-            int r = p0 + 16 * a + (l & 15);
-            r = r < pmax ? r : pmax - 1;
-            po[a] = row_offset(r0 + r);
-        }
-#pragma unroll
-        for (int b = 0; b < TB; ++b) {
-            int r = q0 + 16 * b + (l & 15);
-            r = r < qmax ? r : qmax - 1;
-            qo[b] = row_offset(r0 + r);
-        }
-    }
-
-    struct Frag {
-        bf16x8 ph[TA], pl[TA], qh[TB], ql[TB];
-    };
-
-    __device__ __forceinline__ void load(Frag &f, int s) const
-    {
-        const char *b = (const char *)hp[s >> 2] + 1024 * (s & 3);
-#pragma unroll
-        for (int a = 0; a < TA; ++a) {
-            f.ph[a] = *(const bf16x8 *)(b + po[a]);
-            f.pl[a] = *(const bf16x8 *)(b + po[a] + lo_bytes);
-        }
-#pragma unroll
-        for (int q = 0; q < TB; ++q) {
-            f.qh[q] = *(const bf16x8 *)(b + qo[q]);
-            f.ql[q] = *(const bf16x8 *)(b + qo[q] + lo_bytes);
-        }
-    }
-
-    // the three products, each over all TA x TB tiles before the next (no dependent MFMAs
-    // back to back); small terms first
-    __device__ __forceinline__ void mma(const Frag &f)
-    {
-#pragma unroll
-        for (int a = 0; a < TA; ++a)
-#pragma unroll
-            for (int b = 0; b < TB; ++b) this->acc[a][b] = mfma_bf16(f.ql[b], f.ph[a], this->acc[a][b]);
-#pragma unroll
-        for (int a = 0; a < TA; ++a)
-#pragma unroll
-            for (int b = 0; b < TB; ++b) this->acc[a][b] = mfma_bf16(f.qh[b], f.pl[a], this->acc[a][b]);
-#pragma unroll
-        for (int a = 0; a < TA; ++a)
-#pragma unroll
-            for (int b = 0; b < TB; ++b) this->acc[a][b] = mfma_bf16(f.qh[b], f.ph[a], this->acc[a][b]);
-    }
-
-    template <int S>
-    __device__ __forceinline__ void run_fixed()
-    {
-        Frag buf[U];
-#pragma unroll
-        for (int u = 0; u < D && u < S; ++u) load(buf[u], u);
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (s + D < S) load(buf[(s + D) % U], s + D);
-            mma(buf[s % U]);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-};
-
+// Split BF16 operands, written once per panel when it is finished: a panel slot holds each
+// part of every row in the MFMA operand order -- per group of 16 rows, per k-step s (32
+// columns), per lane quad q, per row, 8 values: element (16 g + rr, s, q, e) at 2048 g + 512 s
+// + 128 q + 8 rr + e -- so a wave's 16-byte operand loads of a 16-row tile read 1 KiB
+// contiguously.  The columns are permuted (the same way for every row, so a product over k is
+// unchanged).  Readers: syrk_tile_split_lds.
 // Writes row R's values v[p][r] (column j = 16 p + 4 r + q of a 128-column panel, q = lane / 16:
-// the layout of trsm_rows' result) to a split slot (hi parts at slot, lo parts lo_off elements
-// later): column j goes to k-step q, lane quad p / 2, position 4 (p % 2) + r -- WaveTileSplit's
-// order.
-__device__ __forceinline__ void split_store_row(__bf16 *slot, int64_t lo_off, int64_t R, const float (&v)[8][4])
+// the layout of trsm_rows' result) to a split slot, part t at slot + t * part_off: column j
+// goes to k-step q, lane quad p / 2, position 4 (p % 2) + r.  PARTS = 2:
+// hi = bf16(x), lo = bf16(x - hi) (16 significant bits); PARTS = 3: hi, mid = bf16(x - hi),
+// lo = bf16(x - hi - mid) (all 24 bits of x; both differences are exact in FP32).
+template <int PARTS = 2>
+__device__ __forceinline__ void split_store_row(__bf16 *slot, int64_t part_off, int64_t R, const float (&v)[8][4])
 {
     const int q = lane_id() >> 4;
     __bf16 *row = slot + (R >> 4) * 2048 + 512 * q + 8 * (R & 15);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        bf16x8 h, lo;
+        bf16x8 part[PARTS];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const float x = v[2 * u + (e >> 2)][e & 3];
-            h[e] = (__bf16)x;
-            lo[e] = (__bf16)(x - (float)h[e]);
+            float x = v[2 * u + (e >> 2)][e & 3];
+#pragma unroll
+            for (int t = 0; t < PARTS; ++t) {
+                const __bf16 h = (__bf16)x;
+                part[t][e] = h;
+                x -= (float)h;
+            }
         }
-        *(bf16x8 *)(row + 128 * u) = h;
-        *(bf16x8 *)(row + 128 * u + lo_off) = lo;
+#pragma unroll
+        for (int t = 0; t < PARTS; ++t) *(bf16x8 *)(row + 128 * u + t * part_off) = part[t];
     }
 }
 
@@ -489,7 +407,9 @@ struct TileSel {
 };
 template <int PREC>
 struct TileSel<float, PREC> {
-    using type = WaveTileF<4, 4, PREC>;
+    // the updates inside a panel step of the 3-part split arithmetic use FP32 MFMAs (the same
+    // 24-bit accuracy; the trailing update reads the split panels instead, syrk_tile_split_lds)
+    using type = WaveTileF<4, 4, PREC == kPrecBF16x6 ? kPrecF32 : PREC>;
 };
 
 // A second FP64 wave tile, for irregular k ranges: TA x TB MFMA tiles, acc[a][b] = tile

@@ -298,13 +298,14 @@ __device__ __forceinline__ void stage_panel_image(double *smem, const double *__
 // float: P and L are read as FP32 and widened, the arithmetic is FP64, the result is rounded
 // back to FP32.  If IMAGE is given, the panel image is copied from there instead of built from
 // L and Xl.  If SPLIT (a split-panel slot) is given (T = float), the rounded rows are also
-// written split into BF16 hi/lo parts, row i as row split_row0 + i of the slot (split_store_row).
+// written split into split_parts BF16 parts (2 or 3), row i as row split_row0 + i of the slot
+// (split_store_row; part t at split + t * split_lo).
 template <bool IDENTITY, typename T>
 __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64_t ldp, int m,
                                           const T *__restrict__ L, int64_t ldl, int bs,
                                           const double *__restrict__ Xl, double *__restrict__ O,
                                           const double *__restrict__ image = nullptr, __bf16 *split = nullptr,
-                                          int64_t split_lo = 0, int64_t split_row0 = 0)
+                                          int64_t split_lo = 0, int64_t split_row0 = 0, int split_parts = 2)
 {
     const int l = lane_id(), w = threadIdx.x >> 6;
     const int row0 = 32 * w;
@@ -354,7 +355,10 @@ __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64
                     for (int p = 0; p < 8; ++p)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) v[p][r] = (float)X[t][p][r];
-                    split_store_row(split, split_lo, split_row0 + i, v);
+                    if (split_parts == 3)
+                        split_store_row<3>(split, split_lo, split_row0 + i, v);
+                    else
+                        split_store_row<2>(split, split_lo, split_row0 + i, v);
                 }
             }
         }
@@ -563,7 +567,7 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__
         global_barrier();  // this role's updated rows are in L2 before other waves read them back
         if (!wait_panel_flag(s_ok, flag, epoch, status) || factor_failed(info)) return;
         trsm_rows<false, T>(smem, Ck, lda, rows, (const T *)nullptr, 0, 128, nullptr, nullptr, Wp, split, split_lo,
-                            R0);
+                            R0, PREC == kPrecBF16x6 ? 3 : 2);
     }
 }
 

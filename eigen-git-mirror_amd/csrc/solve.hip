@@ -39,14 +39,6 @@ constexpr uint32_t kSpinLimit = 1u << 22;  // ~0.2 s of polling
 // Unwritten marker of the packed copies: a signalling NaN (quiet bit 51 clear).
 constexpr unsigned long long kUnwritten = 0x7ff40000000005eeull;
 
-// Workgroup barrier ordering LDS accesses only (outstanding global loads stay in flight).
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 __device__ __forceinline__ bool unwritten(double v) { return (unsigned long long)__double_as_longlong(v) == kUnwritten; }
 
 // BACKWARD = false: forward sweep with L; true: backward sweep with L^T.

@@ -214,7 +214,7 @@ def posv_async_(a: torch.Tensor, b: torch.Tensor, d_info: torch.Tensor, uplo: st
                                  _ld(b), _VP(d_info.data_ptr())), "mi_dposv_async")
 
 
-PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16x3": 3}
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16x3": 3, "bf16x6": 4}
 
 
 def posv_mixed(a: torch.Tensor, b: torch.Tensor, x: torch.Tensor | None = None, prec: str = "fp32",

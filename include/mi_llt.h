@@ -43,7 +43,9 @@ typedef enum {
     MI_PREC_FP64 = 0,  /* FP64 MFMA trailing update (plain path, no refinement needed) */
     MI_PREC_FP32 = 1,  /* FP32 factor storage, FP32 MFMA trailing update, FP32 accumulation */
     MI_PREC_BF16 = 2,  /* FP32 storage, BF16 MFMA trailing update, FP32 accumulation */
-    MI_PREC_BF16X3 = 3 /* FP32 storage, split-BF16 (3 products) trailing update, FP32 acc. */
+    MI_PREC_BF16X3 = 3, /* FP32 storage, split-BF16 (3 products) trailing update, FP32 acc. */
+    MI_PREC_BF16X6 = 4  /* FP32 storage, 3-part split BF16 (6 products, ~FP32 accuracy) trailing
+                           update, FP32 MFMA inside the panel step, FP32 acc. */
 } mi_prec;
 
 /* ---- handle ------------------------------------------------------------------------ */
