@@ -227,7 +227,7 @@ void block_inverses(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, const d
 // Forward + backward substitution in place on B with a factor whose block inverses are in winv.
 template <typename T>
 int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, int64_t ldb, int64_t nrhs,
-               const double *winv, const int *d_info, int sweeps = 3)
+               const double *winv, const int *d_info, int sweeps = 3, bool upper_copy = false)
 {
     int rc = reserve_sync(h, 0);
     if (rc) return rc;
@@ -249,7 +249,7 @@ int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, i
     const double per_sweep = (double)n * (double)n * (double)nrhs;
     traced(h, h->stream, MI_TRACE_SOLVE, per_sweep * ((sweeps & 1) + ((sweeps >> 1) & 1)), [&] {
         mi::launch_solve_persistent<T>(L, ldl, n, B, ldb, nrhs, winv, h->ypk, (int *)h->sync + 1, d_info, (int *)h->sync,
-                                       h->stream, sweeps);
+                                       h->stream, sweeps, upper_copy);
     });
     return 0;
 }
@@ -670,8 +670,10 @@ int mi_dposv_mixed(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const do
     if ((rc = read_back(h, &inf, h->d_info, sizeof(int)))) return rc;
     if (inf != 0) return fp64_path(-1);  // not positive definite in the low-precision factor
     block_inverses<float>(h, m.w32, m.ldw, n, m.wleaf, m.winv, h->d_info);
+    // L^T into the working copy's unused upper triangle, for the backward sweeps of every solve
+    mi::launch_lower_to_upper<float>(m.w32, m.ldw, n, m.w32, m.ldw, s);
     if ((rc = copy_b_to_x())) return rc;
-    if ((rc = solve_only<float>(h, m.w32, m.ldw, n, dX, ldx, nrhs, m.winv, h->d_info))) return rc;
+    if ((rc = solve_only<float>(h, m.w32, m.ldw, n, dX, ldx, nrhs, m.winv, h->d_info, 3, true))) return rc;
     for (int it = 0;; ++it) {
         residual();
         double b = 0.0;
@@ -684,7 +686,7 @@ int mi_dposv_mixed(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const do
         if (it == maxit) break;
         hipError_t e = hipMemcpyAsync(m.D, m.R, (size_t)n * (size_t)nrhs * sizeof(double), hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return fail_hip(h, e, "hipMemcpyAsync");
-        if ((rc = solve_only<float>(h, m.w32, m.ldw, n, m.D, n, nrhs, m.winv, h->d_info))) return rc;
+        if ((rc = solve_only<float>(h, m.w32, m.ldw, n, m.D, n, nrhs, m.winv, h->d_info, 3, true))) return rc;
         mi::launch_add_in_place(dX, ldx, m.D, n, n, nrhs, s);
     }
     return fp64_path(-2);  // refinement did not reach tol

@@ -8,13 +8,14 @@ namespace mi {
 // ------------------------------------------------------------------------------ helpers
 // W (lower) = A (upper)^T through 32x32 LDS tiles; tiles strictly above the diagonal of W
 // are skipped.  Used for uplo = 'U' (LLT.h:377-399 factors the transpose).
-template <bool TO_LOWER>
-__global__ __launch_bounds__(256) void tri_transpose_kernel(const double *__restrict__ S, int64_t lds_,
-                                                            int64_t n, double *__restrict__ D, int64_t ldd)
+// D may be S itself (lower -> its own upper triangle): a tile's reads and writes are separated
+// by the workgroup barrier, and different tiles touch disjoint elements.
+template <bool TO_LOWER, typename T>
+__global__ __launch_bounds__(256) void tri_transpose_kernel(const T *S, int64_t lds_, int64_t n, T *D, int64_t ldd)
 {
     // TO_LOWER: D[i][j] = S[j][i] for i >= j (D lower from S upper)
     // else    : D[j][i] = S[i][j] for i >= j (D upper from S lower)
-    __shared__ double t[32][33];
+    __shared__ T t[32][33];
     const int bi = blockIdx.x, bj = blockIdx.y;  // tile of the lower triangle (rows bi, cols bj)
     if (bj > bi) return;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -36,14 +37,17 @@ __global__ __launch_bounds__(256) void tri_transpose_kernel(const double *__rest
 void launch_upper_to_lower(const double *A, int64_t lda, int64_t n, double *W, int64_t ldw, hipStream_t s)
 {
     const unsigned T = (unsigned)((n + 31) / 32);
-    hipLaunchKernelGGL(tri_transpose_kernel<true>, dim3(T, T), dim3(256), 0, s, A, lda, n, W, ldw);
+    hipLaunchKernelGGL((tri_transpose_kernel<true, double>), dim3(T, T), dim3(256), 0, s, A, lda, n, W, ldw);
 }
 
-void launch_lower_to_upper(const double *W, int64_t ldw, int64_t n, double *A, int64_t lda, hipStream_t s)
+template <typename T>
+void launch_lower_to_upper(const T *W, int64_t ldw, int64_t n, T *A, int64_t lda, hipStream_t s)
 {
-    const unsigned T = (unsigned)((n + 31) / 32);
-    hipLaunchKernelGGL(tri_transpose_kernel<false>, dim3(T, T), dim3(256), 0, s, W, ldw, n, A, lda);
+    const unsigned G = (unsigned)((n + 31) / 32);
+    hipLaunchKernelGGL((tri_transpose_kernel<false, T>), dim3(G, G), dim3(256), 0, s, W, ldw, n, A, lda);
 }
+template void launch_lower_to_upper<double>(const double *, int64_t, int64_t, double *, int64_t, hipStream_t);
+template void launch_lower_to_upper<float>(const float *, int64_t, int64_t, float *, int64_t, hipStream_t);
 
 // Symmetric L1 norm (LLT.h:442-453).  For lower storage column c sums |A(i,c)|, i >= c
 // (col(c).tail) plus |A(c,j)|, j < c (row(c).head); upper storage mirrors it.  The column

@@ -58,14 +58,19 @@ void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hip
 // scratch (each finished block's packed copy for its consumers, per sweep).
 // sweeps: 1 forward only (L Y = B), 2 backward only (L^T X = B), 3 both.
 template <typename T>
+// upper_copy: L's strictly upper triangle holds L^T (launch_lower_to_upper), which the backward
+// sweep then reads with the forward sweep's coalesced access.
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, double *Ypk, int *tickets, const int *info, int *status, hipStream_t s,
-                             int sweeps = 3);
+                             int sweeps = 3, bool upper_copy = false);
 // Doubles of one sweep's packed block copies for n rows and nrhs right-hand sides.
 inline int64_t solve_pack_doubles(int64_t n, int64_t nrhs) { return ((n + 127) / 128) * ((nrhs + 15) / 16) * 2048; }
 // Symmetric-storage helpers.
 void launch_upper_to_lower(const double *A, int64_t lda, int64_t n, double *W, int64_t ldw, hipStream_t s);
-void launch_lower_to_upper(const double *W, int64_t ldw, int64_t n, double *A, int64_t lda, hipStream_t s);
+// A's upper triangle = W's lower triangle transposed (A may be W: the copy into its own upper
+// triangle that the backward solve reads with upper_copy).
+template <typename T>
+void launch_lower_to_upper(const T *W, int64_t ldw, int64_t n, T *A, int64_t lda, hipStream_t s);
 void launch_sy_l1norm(const double *A, int64_t lda, int64_t n, bool upper, double *colsum, double *out,
                       hipStream_t s);
 

@@ -58,7 +58,11 @@ __device__ __forceinline__ bool unwritten(double v) { return (unsigned long long
 // Each wave owns one 16-row tile of the block, rt = w for waves 0..3 and 11 - w for waves
 // 4..7: waves w and w+4 share a SIMD, and the triangular X_i product costs row tile rt
 // 4(rt+1) MFMA steps forward (4(8-rt) backward), so every SIMD issues 36 of them.
-template <bool BACKWARD, typename T>
+//
+// UC (backward only): L's strictly upper triangle holds L^T, so the operand L_ji^T of block i
+// is read as the upper block (i, j) -- lanes over consecutive rows, like the forward sweep --
+// instead of across 16 columns of L per load.
+template <bool BACKWARD, typename T, bool UC = false>
 __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ L, int64_t lda, int64_t n,
                                                           double *__restrict__ B, int64_t ldb, int64_t nrhs,
                                                           const double *__restrict__ Winv, double *Ypk, int *ticket,
@@ -160,11 +164,11 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
     auto aim = [&](int jb) {
         const int64_t j0 = (int64_t)jb * SB;
         bjm = (int)((n - j0) < SB ? (n - j0) : SB) - 1;
-        lb = BACKWARD ? L + j0 + (i0 + rc) * lda : L + (i0 + rc) + j0 * lda;
+        lb = (BACKWARD && !UC) ? L + j0 + (i0 + rc) * lda : L + (i0 + rc) + j0 * lda;
     };
     auto operand = [&](int s) -> double {
         const int k = (4 * s + kl) < bjm ? (4 * s + kl) : bjm;
-        return (double)lb[BACKWARD ? (int64_t)k : (int64_t)k * lda];
+        return (double)lb[(BACKWARD && !UC) ? (int64_t)k : (int64_t)k * lda];
     };
     // op(X_i): column-major P[row][k] = X_i[row][k] (forward, lower: k < 16(rt+1)) or X_i[k][row]
     // (backward, upper: k >= 16 rt); only the nonzero k-steps are loaded and multiplied
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(256) void solve_reset_kernel(int *ticket, double *Y
 template <typename T>
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, double *Ypk, int *tickets, const int *info, int *status, hipStream_t s,
-                             int sweeps)
+                             int sweeps, bool upper_copy)
 {
     const int nblk = (int)((n + SB - 1) / SB);
     const int ngrp = (int)((nrhs + SC - 1) / SC);
@@ -295,14 +299,20 @@ void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int6
     }
     if (sweeps & 2) {
         hipLaunchKernelGGL(solve_reset_kernel, dim3(fill), dim3(256), 0, s, tickets + 1, Ypk + pk, pk);
-        hipLaunchKernelGGL((solve_sweep_kernel<true, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv,
-                           Ypk + pk, tickets + 1, info, status);
+        if (upper_copy)
+            hipLaunchKernelGGL((solve_sweep_kernel<true, T, true>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb,
+                               nrhs, Winv, Ypk + pk, tickets + 1, info, status);
+        else
+            hipLaunchKernelGGL((solve_sweep_kernel<true, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs,
+                               Winv, Ypk + pk, tickets + 1, info, status);
     }
 }
 
 template void launch_solve_persistent<double>(const double *, int64_t, int64_t, double *, int64_t, int64_t,
-                                              const double *, double *, int *, const int *, int *, hipStream_t, int);
+                                              const double *, double *, int *, const int *, int *, hipStream_t, int,
+                                              bool);
 template void launch_solve_persistent<float>(const float *, int64_t, int64_t, double *, int64_t, int64_t,
-                                             const double *, double *, int *, const int *, int *, hipStream_t, int);
+                                             const double *, double *, int *, const int *, int *, hipStream_t, int,
+                                             bool);
 
 }  // namespace mi
