@@ -230,7 +230,7 @@ template void launch_sym_copy_lower<double>(const double *, int64_t, int64_t, bo
 int residual_chunks(int64_t n, int64_t nrhs)
 {
     const int64_t wg = ((n + 63) / 64) * ((nrhs + 15) / 16);
-    int64_t c = (2048 + wg - 1) / wg;
+    int64_t c = (2560 + wg - 1) / wg;  // two full rounds of 5 workgroups on each of 256 CUs
     c = c < 1 ? 1 : (c > 16 ? 16 : c);
     return (int)c;
 }
