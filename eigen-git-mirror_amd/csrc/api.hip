@@ -416,7 +416,11 @@ int mi_llt_set_stream(mi_llt_handle h, void *stream)
 
 const char *mi_llt_last_error(mi_llt_handle h) { return valid(h) ? h->err : "invalid handle"; }
 
-int64_t mi_llt_workspace_bytes(mi_llt_handle h) { return valid(h) ? (int64_t)h->ws_bytes : -1; }
+int64_t mi_llt_workspace_bytes(mi_llt_handle h)
+{
+    if (!valid(h)) return -1;
+    return (int64_t)(h->ws_bytes + h->ypk_bytes + h->vb_bytes + h->sync_words * sizeof(uint32_t));
+}
 
 int mi_llt_status(mi_llt_handle h, int *status)
 {

@@ -55,7 +55,8 @@ int mi_llt_destroy(mi_llt_handle h);
 int mi_llt_set_stream(mi_llt_handle h, void *stream);
 /* Text of the last runtime failure on this handle ("" if none). */
 const char *mi_llt_last_error(mi_llt_handle h);
-/* Device bytes currently held by the handle's workspace. */
+/* Device bytes currently held by the handle: its workspace, the solve's scratch, the batched
+   workspace and the synchronisation words (all grow-only, freed by mi_llt_destroy). */
 int64_t mi_llt_workspace_bytes(mi_llt_handle h);
 /* Waits for the handle's stream, then returns (and clears) the device status bits of the
  * asynchronous calls since the last query: 0, or a combination of MI_LLT_STATUS_SOLVE_TIMEOUT
@@ -155,7 +156,8 @@ int mi_llt_get_vbatched_config(mi_llt_handle h, int *wg_per_cu, int *group, int 
  *   dloss/dA on return, as a full symmetric matrix (both triangles written; the symmetric
  *   convention of torch.linalg.cholesky): Abar = sym(L^{-T} M L^{-1}), M the halved symmetric
  *   completion of tril(L^T Lbar).  The diagonal-block inverses of L are recomputed, not
- *   stored by the forward pass.  Device workspace: 2 n^2 doubles.  Asynchronous.
+ *   stored by the forward pass.  Device workspace: 3 n^2 doubles plus 0.75 MiB per 128 rows
+ *   (the diagonal-block inverses and couplings).  Asynchronous.
  * mi_dpotrs_adjoint: for X = A^{-1} B computed with the factor L: dXbar holds Xbar on entry
  *   and Bbar = A^{-1} Xbar on return; dLbar (lower triangle, ld ldlb) is accumulated:
  *   Lbar -= tril(Bbar (L^T X)^T + X (L^T Bbar)^T).  Asynchronous. */
