@@ -218,10 +218,10 @@ int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, i
 // Diagonal-block inverses of a factor (needs its leaf inverses), for solve_only.
 template <typename T>
 void block_inverses(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, const double *wleaf, double *winv,
-                    const int *d_info)
+                    const int *d_info, bool couplings = true)
 {
     traced(h, h->stream, MI_TRACE_TRTRI, (double)n * mi::kNB * mi::kNB / 3.0,
-           [&] { mi::launch_block_inverses<T>(L, ldl, n, wleaf, winv, d_info, h->stream); });
+           [&] { mi::launch_block_inverses<T>(L, ldl, n, wleaf, winv, d_info, h->stream, couplings); });
 }
 
 // Forward + backward substitution in place on B with a factor whose block inverses are in winv.
@@ -758,7 +758,7 @@ int mi_dpotrf_adjoint(mi_llt_handle h, char uplo, int64_t n, const double *dL, i
     traced(h, s, MI_TRACE_GEMM, nd * nd * nd / 3.0, [&] { mi::launch_trmm_tn_lower(Wt, n, W3, n, n, W3, n, W2, s); });
     mi::launch_sym_tile(0, W3, n, n, W2, n, s);              // M
     mi::launch_leaf_inverses<double>(dL, ldl, n, wleaf, s);  // recomputed from L
-    block_inverses<double>(h, dL, ldl, n, wleaf, winv, nullptr);
+    block_inverses<double>(h, dL, ldl, n, wleaf, winv, nullptr, false);  // no solve here
     traced(h, s, MI_TRACE_SOLVE, nd * nd * nd,
            [&] { mi::launch_trsm_xl_multi(Wt, n, n, W2, n, n, winv, false, s); });  // Y = M L^{-1}
     mi::launch_sym_tile(1, W2, n, n, W3, n, s);                                      // Y^T = L^{-T} M

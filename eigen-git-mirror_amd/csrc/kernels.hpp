@@ -47,9 +47,10 @@ void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, dou
 __host__ __device__ inline int64_t split_part_elems(int64_t n) { return ((n + 127) / 128 * 128) * 128; }
 __host__ __device__ inline int64_t split_slot_elems(int64_t n, int parts) { return parts * split_part_elems(n); }
 // Winv[b] = {X_b^T, X_b, M_b, N_b, M2_b, N2_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).
+// couplings = false: X_b^T and X_b only (for users of Winv other than the solve).
 template <typename T>
 void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv, const int *info,
-                           hipStream_t s);
+                           hipStream_t s, bool couplings = true);
 // Wleaf from an already factored matrix (standalone solve).
 template <typename T>
 void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s);

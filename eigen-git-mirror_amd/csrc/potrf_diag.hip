@@ -142,11 +142,11 @@ __global__ __launch_bounds__(256) void leaf_inverse_kernel(const T *__restrict__
 
 template <typename T>
 void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv, const int *info,
-                           hipStream_t s)
+                           hipStream_t s, bool couplings)
 {
     const unsigned nblk = (unsigned)((n + 127) / 128);
     hipLaunchKernelGGL(block_inverse_kernel<T>, dim3(nblk), dim3(256), 0, s, L, lda, n, Wleaf, Winv, info);
-    if (nblk > 1)
+    if (couplings && nblk > 1)
         hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(4 * nblk), dim3(256), 0, s, L, lda, n, Winv, info);
 }
 
@@ -158,9 +158,9 @@ void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hip
 }
 
 template void launch_block_inverses<double>(const double *, int64_t, int64_t, const double *, double *, const int *,
-                                            hipStream_t);
+                                            hipStream_t, bool);
 template void launch_block_inverses<float>(const float *, int64_t, int64_t, const double *, double *, const int *,
-                                           hipStream_t);
+                                           hipStream_t, bool);
 template void launch_leaf_inverses<double>(const double *, int64_t, int64_t, double *, hipStream_t);
 template void launch_leaf_inverses<float>(const float *, int64_t, int64_t, double *, hipStream_t);
 
