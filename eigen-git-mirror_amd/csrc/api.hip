@@ -81,12 +81,19 @@ void traced(mi_llt_handle h, hipStream_t s, int kind, double flops, F &&launch)
         return;
     }
     mi_llt_context::Rec r{kind, flops, nullptr, nullptr};
-    hipEventCreate(&r.e0);
-    hipEventCreate(&r.e1);
-    hipEventRecord(r.e0, s);
+    if (hipEventCreate(&r.e0) != hipSuccess || hipEventCreate(&r.e1) != hipSuccess) {
+        if (r.e0) (void)hipEventDestroy(r.e0);
+        launch();  // untimed: tracing is best effort
+        return;
+    }
+    const bool rec = hipEventRecord(r.e0, s) == hipSuccess;
     launch();
-    hipEventRecord(r.e1, s);
-    h->recs.push_back(r);
+    if (rec && hipEventRecord(r.e1, s) == hipSuccess) {
+        h->recs.push_back(r);
+    } else {
+        (void)hipEventDestroy(r.e0);
+        (void)hipEventDestroy(r.e1);
+    }
 }
 
 size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -97,7 +104,7 @@ int reserve(mi_llt_handle h, size_t bytes)
     if (bytes <= h->ws_bytes) return 0;
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
-    if (h->ws) hipFree(h->ws);
+    if (h->ws) (void)hipFree(h->ws);
     h->ws = nullptr;
     h->ws_bytes = 0;
     e = hipMalloc(&h->ws, bytes);
@@ -146,7 +153,7 @@ int reserve_sync(mi_llt_handle h, size_t flag_words)
     if (need <= h->sync_words) return 0;
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
-    if (h->sync) hipFree(h->sync);
+    if (h->sync) (void)hipFree(h->sync);
     h->sync = nullptr;
     h->sync_words = 0;
     const size_t words = need < 4096 ? 4096 : need;
@@ -235,7 +242,7 @@ int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, i
     if (pk > h->ypk_bytes) {
         hipError_t e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
-        if (h->ypk) hipFree(h->ypk);
+        if (h->ypk) (void)hipFree(h->ypk);
         h->ypk = nullptr;
         h->ypk_bytes = 0;
         e = hipMalloc(&h->ypk, pk);
@@ -344,7 +351,7 @@ int vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n, T *const
     if (need > h->vb_bytes) {
         hipError_t e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
-        if (h->vb_ws) hipFree(h->vb_ws);
+        if (h->vb_ws) (void)hipFree(h->vb_ws);
         h->vb_ws = nullptr;
         h->vb_bytes = 0;
         if (hipMalloc(&h->vb_ws, need) != hipSuccess) {
@@ -382,7 +389,7 @@ int mi_llt_create(mi_llt_handle *h)
     if (!h) return -1;
     mi_llt_context *c = new (std::nothrow) mi_llt_context();
     if (!c) return MI_LLT_E_ALLOC;
-    hipGetDevice(&c->device);
+    if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
     if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || c->cus <= 0)
         c->cus = 256;
     if (hipMalloc(&c->d_info, 256) != hipSuccess) {
@@ -396,12 +403,12 @@ int mi_llt_create(mi_llt_handle *h)
 int mi_llt_destroy(mi_llt_handle h)
 {
     if (!valid(h)) return MI_LLT_E_HANDLE;
-    hipStreamSynchronize(h->stream);
-    if (h->sync) hipFree(h->sync);
-    if (h->ypk) hipFree(h->ypk);
-    if (h->vb_ws) hipFree(h->vb_ws);
-    if (h->ws) hipFree(h->ws);
-    if (h->d_info) hipFree(h->d_info);
+    (void)hipStreamSynchronize(h->stream);  // destroying: nothing to report an error to
+    if (h->sync) (void)hipFree(h->sync);
+    if (h->ypk) (void)hipFree(h->ypk);
+    if (h->vb_ws) (void)hipFree(h->vb_ws);
+    if (h->ws) (void)hipFree(h->ws);
+    if (h->d_info) (void)hipFree(h->d_info);
     h->magic = 0;
     delete h;
     return 0;
@@ -454,12 +461,13 @@ int mi_llt_trace_read(mi_llt_handle h, int kind, double *ms, double *flops, int6
         if (e != hipSuccess) return fail_hip(h, e, "trace sync");
         for (auto &r : h->recs) {
             float t = 0.f;
-            hipEventElapsedTime(&t, r.e0, r.e1);
-            h->t_ms[r.kind] += t;
-            h->t_flops[r.kind] += r.flops;
-            h->t_calls[r.kind] += 1;
-            hipEventDestroy(r.e0);
-            hipEventDestroy(r.e1);
+            if (hipEventElapsedTime(&t, r.e0, r.e1) == hipSuccess) {
+                h->t_ms[r.kind] += t;
+                h->t_flops[r.kind] += r.flops;
+                h->t_calls[r.kind] += 1;
+            }
+            (void)hipEventDestroy(r.e0);
+            (void)hipEventDestroy(r.e1);
         }
         h->recs.clear();
     }

@@ -25,7 +25,7 @@ struct ThreadState {
     size_t bytes = 0;
     ~ThreadState()
     {
-        if (buf) hipFree(buf);
+        if (buf) (void)hipFree(buf);
         if (h) mi_llt_destroy(h);
     }
 };
@@ -36,7 +36,7 @@ bool ensure(size_t bytes)
 {
     if (!tls.h && mi_llt_create(&tls.h) != 0) return false;
     if (bytes <= tls.bytes) return true;
-    if (tls.buf) hipFree(tls.buf);
+    if (tls.buf) (void)hipFree(tls.buf);
     tls.buf = nullptr;
     tls.bytes = 0;
     if (hipMalloc(&tls.buf, bytes) != hipSuccess) return false;
