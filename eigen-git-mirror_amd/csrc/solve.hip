@@ -69,7 +69,10 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
                                                           const int *info, int *status)
 {
     if (factor_failed(info)) return;  // uniform: info is final before this kernel starts
-    __shared__ double Ys[SP];         // Ys[c + SC*k]: the published block being consumed
+    // Ys[c + SC*k + 16 (k / 32)]: the published block being consumed, one padding row per
+    // 32 k, so that lanes reading k = 32 q + s (q = lane / 16, the permuted order below) hit
+    // distinct banks
+    __shared__ double Ys[SP + 4 * 16];
     __shared__ double Ts[SP];         // Ts[c + SC*r]: the block's residual before the inverse
     __shared__ int s_item, s_ok, s_fail;
     const int nblk = (int)((n + SB - 1) / SB);
@@ -151,7 +154,7 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
                     break;
                 }
             }
-            Ys[e * 512 + tid] = v[e];
+            Ys[e * 528 + tid] = v[e];
         }
         lds_barrier();
         return s_fail == 0;
@@ -159,6 +162,11 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
 
     // P operand of a contributing block, k-step s: P[row][k] = L_ij[row][k] (forward) or
     // L_ji[k][row] (backward); k clamped in bounds.  `aim` points it at block jb.
+    // PERM (FP64 backward without the upper copy): there the operand is a column of L, and a
+    // lane takes k = 32 q + s at step s (q = lane / 16) instead of 4 s + q -- the 32 k of a lane
+    // are contiguous in memory and load as 16-byte pairs (when lda is even and the block is
+    // full), not as 32 loads that each touch 16 columns.  The sum over k is the same.
+    constexpr bool PERM = BACKWARD && !UC && sizeof(T) == 8;
     const T *lb = nullptr;  // this lane's row (forward) or column (backward) of the block
     int bjm = SB - 1;       // last valid k of the block
     auto aim = [&](int jb) {
@@ -166,10 +174,26 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         bjm = (int)((n - j0) < SB ? (n - j0) : SB) - 1;
         lb = (BACKWARD && !UC) ? L + j0 + (i0 + rc) * lda : L + (i0 + rc) + j0 * lda;
     };
-    auto operand = [&](int s) -> double {
-        const int k = (4 * s + kl) < bjm ? (4 * s + kl) : bjm;
-        return (double)lb[(BACKWARD && !UC) ? (int64_t)k : (int64_t)k * lda];
+    auto kof = [&](int s) -> int { return PERM ? 32 * kl + s : 4 * s + kl; };
+    auto load_operand = [&](double (&dst)[SB / 4]) {
+        if (PERM && (lda & 1) == 0 && bjm == SB - 1) {  // 16-byte aligned: lda even, j0 % 128 == 0
+#pragma unroll
+            for (int u = 0; u < SB / 8; ++u) {
+                typedef double d2 __attribute__((ext_vector_type(2)));
+                const d2 d = *(const d2 *)((const double *)lb + 32 * kl + 2 * u);
+                dst[2 * u] = d[0];
+                dst[2 * u + 1] = d[1];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < SB / 4; ++s) {
+                const int k = kof(s) < bjm ? kof(s) : bjm;
+                dst[s] = (double)lb[(BACKWARD && !UC) ? (int64_t)k : (int64_t)k * lda];
+            }
+        }
     };
+    // Ys element (lane's column, k)
+    auto ys = [&](int k) -> double { return Ys[(l & 15) + SC * k + 16 * (k >> 5)]; };
     // op(X_i): column-major P[row][k] = X_i[row][k] (forward, lower: k < 16(rt+1)) or X_i[k][row]
     // (backward, upper: k >= 16 rt); only the nonzero k-steps are loaded and multiplied
     const double *Xp = Wi + (BACKWARD ? 0 : kInvX) + r + SB * kl;
@@ -182,8 +206,7 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
     const bool contrib = jb != jend;  // any plain contributions (before the two coupled blocks)
     if (contrib) {
         aim(jb);
-#pragma unroll
-        for (int s = 0; s < SB / 4; ++s) pv[s] = operand(s);
+        load_operand(pv);
     }
     for (; jb != jend; jb += jstep) {
         if (!stage(jb)) return;
@@ -193,14 +216,13 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         // MFMA issues, so X_i is in flight during these MFMAs.
         if (jb + jstep != jend) {
 #pragma unroll
-            for (int s = 0; s < SB / 4; ++s) acc = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], acc);
+            for (int s = 0; s < SB / 4; ++s) acc = mfma_f64(ys(kof(s)), pv[s], acc);
             aim(jb + jstep);
-#pragma unroll
-            for (int s = 0; s < SB / 4; ++s) pv[s] = operand(s);  // in flight during the next wait
+            load_operand(pv);  // in flight during the next wait
         } else {
 #pragma unroll
             for (int s = 0; s < SB / 4; ++s) {
-                acc = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], acc);
+                acc = mfma_f64(ys(kof(s)), pv[s], acc);
                 if (s >= s_lo && s < s_hi) pv[s] = Xp[(size_t)4 * SB * s];
             }
         }
@@ -239,7 +261,7 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
         for (int q = 0; q < 4; ++q) cq[q] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < SB / 4; ++s) {
-            cq[s & 3] = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], cq[s & 3]);
+            cq[s & 3] = mfma_f64(ys(4 * s + kl), pv[s], cq[s & 3]);
             pv[s] = M1p[(size_t)4 * SB * s];
         }
 #pragma unroll
@@ -250,7 +272,7 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
 #pragma unroll
         for (int q = 0; q < 4; ++q) cq[q] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s = 0; s < SB / 4; ++s) cq[s & 3] = mfma_f64(Ys[(l & 15) + SC * (4 * s + kl)], pv[s], cq[s & 3]);
+        for (int s = 0; s < SB / 4; ++s) cq[s & 3] = mfma_f64(ys(4 * s + kl), pv[s], cq[s & 3]);
 #pragma unroll
         for (int e = 0; e < 4; ++e) y[e] -= (cq[0][e] + cq[1][e]) + (cq[2][e] + cq[3][e]);
     }
