@@ -20,11 +20,17 @@ __global__ __launch_bounds__(256) void tri_transpose_kernel(const T *S, int64_t 
     if (bj > bi) return;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     const int64_t i0 = (int64_t)bi * 32, j0 = (int64_t)bj * 32;
-    // read source block: TO_LOWER reads S[j0+.., i0+..] (upper), else S[i0+.., j0+..] (lower)
-    for (int y = ty; y < 32; y += 8) {
+    // read source block: TO_LOWER reads S[j0+.., i0+..] (upper), else S[i0+.., j0+..] (lower);
+    // a thread's four loads are issued before its LDS stores (one memory round trip)
+    T v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int y = ty + 8 * q;
         const int64_t rr = (TO_LOWER ? j0 : i0) + tx, cc = (TO_LOWER ? i0 : j0) + y;
-        if (rr < n && cc < n) t[y][tx] = S[rr + cc * lds_];
+        v[q] = (rr < n && cc < n) ? S[rr + cc * lds_] : T(0);
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t[ty + 8 * q][tx] = v[q];
     __syncthreads();
     for (int y = ty; y < 32; y += 8) {
         const int64_t rr = (TO_LOWER ? i0 : j0) + tx, cc = (TO_LOWER ? j0 : i0) + y;

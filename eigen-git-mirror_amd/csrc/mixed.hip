@@ -33,12 +33,22 @@ __global__ __launch_bounds__(256) void sym_copy_lower_kernel(const double *__res
         }
         __syncthreads();
     }
-    for (int y = ty; y < 64; y += 4) {
+    // all 16 elements of a thread first (one memory round trip), then the stores
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int y = ty + 4 * q;
+        const int64_t i = i0 + tx, j = j0 + y;
+        const bool in = i < n && j < n && i >= j;
+        v[q] = upper ? tile[tx][y] : (in ? A[i + j * lda] : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int y = ty + 4 * q;
         const int64_t i = i0 + tx, j = j0 + y;
         if (i < n && j < n && i >= j) {
-            const double v = upper ? tile[tx][y] : A[i + j * lda];
-            W[i + j * ldw] = (T)v;
-            s += (i == j ? 1.0 : 2.0) * v * v;
+            W[i + j * ldw] = (T)v[q];
+            s += (i == j ? 1.0 : 2.0) * v[q] * v[q];
         }
     }
     red[threadIdx.x] = s;
