@@ -109,22 +109,32 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(const double *__restrict_
     __shared__ double t[32][33];
     const int64_t bi = blockIdx.x, bj = blockIdx.y;  // output tile rows bi, cols bj
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    // mirrored tile in[bj.., bi..] -> t[x][y] = in[bj*32 + x][bi*32 + y]
-    for (int y = ty; y < 32; y += 8) {
+    // mirrored tile in[bj.., bi..] -> t[x][y] = in[bj*32 + x][bi*32 + y]; each thread issues all
+    // its loads (the mirrored ones and, except for modes 1 and 4, its own) before using them
+    double m[4], d[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int y = ty + 8 * q;
         const int64_t r = bj * 32 + tx, c = bi * 32 + y;
-        t[tx][y] = (r < n && c < n) ? in[r + c * ldi] : 0.0;
+        m[q] = (r < n && c < n) ? in[r + c * ldi] : 0.0;
+        const int64_t i = bi * 32 + tx, j = bj * 32 + y;
+        d[q] = (MODE != 1 && MODE != 4 && i < n && j < n) ? in[i + j * ldi] : 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t[tx][ty + 8 * q] = m[q];
     __syncthreads();
-    for (int y = ty; y < 32; y += 8) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int y = ty + 8 * q;
         const int64_t i = bi * 32 + tx, j = bj * 32 + y;
         if (i >= n || j >= n) continue;
         const double tr = t[y][tx];  // in[j][i]
         double v;
-        if (MODE == 0) v = 0.5 * (i >= j ? in[i + j * ldi] : tr);
+        if (MODE == 0) v = 0.5 * (i >= j ? d[q] : tr);
         else if (MODE == 1) v = tr;
-        else if (MODE == 3 && (i >> 7) != (j >> 7)) v = i < j ? in[i + j * ldi] : tr;
+        else if (MODE == 3 && (i >> 7) != (j >> 7)) v = i < j ? d[q] : tr;
         else if (MODE == 4) v = j >= i ? tr : 0.0;
-        else v = 0.5 * (in[i + j * ldi] + tr);
+        else v = 0.5 * (d[q] + tr);
         out[i + j * ldo] = v;
     }
 }
