@@ -11,6 +11,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <math.h>
+
+#include <limits>
 #include <new>
 #include <vector>
 
@@ -25,13 +28,14 @@ struct mi_llt_context {
     void *ws = nullptr;       // grow-only device workspace
     size_t ws_bytes = 0;
     int *d_info = nullptr;    // device int for the synchronous entry points
-    // persistent synchronisation words: [status | 2 solve tickets | - | factor tickets | panel flag | ...]
+    // synchronisation words: [status | 2 solve tickets | - | factor tickets | panel flag | ...];
+    // zeroed on the handle's stream when allocated; the ticket and flag words of a solve or a
+    // factorization are reset on the stream at its start, so every call (and every replay of a
+    // captured call) counts from zero
     uint32_t *sync = nullptr;
     size_t sync_words = 0;
     double *ypk = nullptr;       // the solve's packed copies of finished blocks (grow-only)
     size_t ypk_bytes = 0;
-    uint32_t panel_epoch = 0;    // panel-step flag
-    uint32_t panel_tickets = 0;  // tickets drawn so far from the panel ticket counter
     char err[256] = {0};
     uint32_t magic = 0x4c4c5431u;
     // optional per-kernel timing (mi_llt_trace_*): event pairs around each launch
@@ -56,6 +60,12 @@ struct mi_llt_context {
 namespace {
 
 __global__ void set_int_kernel(int *p, int v) { *p = v; }
+__global__ void set_double_kernel(double *p, double v) { *p = v; }
+__global__ void or_int_kernel(int *p, int v) { atomicOr(p, v); }
+__global__ void zero_words_kernel(uint32_t *p, int count)
+{
+    for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0u;
+}
 
 int fail_hip(mi_llt_handle h, hipError_t e, const char *what)
 {
@@ -157,12 +167,16 @@ int reserve_sync(mi_llt_handle h, size_t flag_words)
     h->sync = nullptr;
     h->sync_words = 0;
     const size_t words = need < 4096 ? 4096 : need;
-    if (hipMalloc(&h->sync, words * sizeof(uint32_t)) != hipSuccess) return MI_LLT_E_ALLOC;
-    e = hipMemset(h->sync, 0, words * sizeof(uint32_t));
-    if (e != hipSuccess) return fail_hip(h, e, "hipMemset");
+    if (hipMalloc(&h->sync, words * sizeof(uint32_t)) != hipSuccess) {
+        h->sync = nullptr;
+        return MI_LLT_E_ALLOC;
+    }
+    // zeroed on the handle's own stream, so that the kernels queued after it on that stream see
+    // the zeros (a memset on the null stream is not ordered with a non-blocking stream)
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(256), 0, h->stream, h->sync, (int)words);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail_hip(h, e, "zero sync words");
     h->sync_words = words;
-    h->panel_epoch = 0;
-    h->panel_tickets = 0;
     return 0;
 }
 
@@ -193,12 +207,12 @@ int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, i
     const int64_t nblk = (n + NB - 1) / NB;
     int rc = reserve_sync(h, 0);
     if (rc) return rc;
-    if (h->panel_epoch > 0x7fff0000u) {  // the flag only grows: restart it from zero
-        hipError_t e = hipMemsetAsync(h->sync + 5, 0, sizeof(uint32_t), h->stream);
-        if (e != hipSuccess) return fail_hip(h, e, "hipMemsetAsync");
-        h->panel_epoch = 0;
-    }
     hipStream_t s = h->stream;
+    // ticket counter and panel flag start from zero for every factorization (a kernel, not a
+    // memset: the factorization may be captured in a graph); the ticket bases and the flag
+    // epochs below count from there, so they are the same in every call and every replay
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, h->sync + 4, 2);
+    uint32_t tickets = 0, epoch_now = 0;
     for (int64_t k = -1; k < nblk; ++k) {
         const int64_t kp = (k + 1 < nblk) ? (k + 1) * NB : -1;  // panel of this launch
         // trailing update of launch k: panels k-1, k (k >= 1; panel 0 alone for k = 0) to the
@@ -211,13 +225,13 @@ int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, i
         }
         if (kp < 0 && fl_upd == 0.0) break;
         const double fl = (kp >= 0 ? panel_flops(n, kp, NB) : 0.0) + fl_upd;
-        const uint32_t epoch = (kp >= 0) ? ++h->panel_epoch : h->panel_epoch;
+        const uint32_t epoch = (kp >= 0) ? ++epoch_now : epoch_now;
         uint32_t grid = 0;
         traced(h, s, MI_TRACE_POTRF, fl, [&] {
-            mi::launch_factor_step<T>(A, lda, n, kp, k, wleaf, d_info, h->sync + 4, h->panel_tickets, h->sync + 5,
-                                      epoch, (int *)h->sync, s, &grid, prec, wsplit);
+            mi::launch_factor_step<T>(A, lda, n, kp, k, wleaf, d_info, h->sync + 4, tickets, h->sync + 5, epoch,
+                                      (int *)h->sync, s, &grid, prec, wsplit);
         });
-        h->panel_tickets += grid;
+        tickets += grid;
     }
     return 0;
 }
@@ -373,11 +387,53 @@ int vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n, T *const
     return check_launch(h, "vbatched");
 }
 
+// Waits for the stream, then reads and clears the status word (0 when none is allocated yet).
+int read_status(mi_llt_handle h, int *status)
+{
+    *status = 0;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
+    if (!h->sync) return 0;
+    int v = 0;
+    e = hipMemcpyAsync(&v, h->sync, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(h, e, "status readback");
+    if (v != 0) {  // cleared on the stream: ordered before any later work of the handle
+        hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, (int *)h->sync, 0);
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail_hip(h, e, "status clear");
+    }
+    *status = v;
+    return 0;
+}
+
+// MI_LLT_E_DEVICE with the status bits spelled out in the handle's error text.
+int device_failure(mi_llt_handle h, int bits)
+{
+    snprintf(h->err, sizeof h->err, "device status %d:%s%s%s", bits,
+             (bits & MI_LLT_STATUS_SOLVE_TIMEOUT) ? " solve sweep timed out;" : "",
+             (bits & MI_LLT_STATUS_PANEL_TIMEOUT) ? " panel solve timed out waiting for its diagonal block;" : "",
+             (bits & MI_LLT_STATUS_FACTOR_TICKET) ? " factorization ticket outside its launch;" : "");
+    return MI_LLT_E_DEVICE;
+}
+
+// Waits for the stream and reports a device failure recorded since the last check.
+int check_device(mi_llt_handle h)
+{
+    int st = 0;
+    int rc = read_status(h, &st);
+    if (rc) return rc;
+    return st ? device_failure(h, st) : 0;
+}
+
+// Synchronous copy to the host of a result on the stream; fails on a device failure recorded
+// in the status word (the result is then invalid).
 int read_back(mi_llt_handle h, void *dst, const void *src, size_t bytes)
 {
     hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    return e == hipSuccess ? 0 : fail_hip(h, e, "readback");
+    if (e != hipSuccess) return fail_hip(h, e, "readback");
+    return check_device(h);
 }
 
 }  // namespace
@@ -417,7 +473,21 @@ int mi_llt_destroy(mi_llt_handle h)
 int mi_llt_set_stream(mi_llt_handle h, void *stream)
 {
     if (!valid(h)) return MI_LLT_E_HANDLE;
-    h->stream = (hipStream_t)stream;
+    hipStream_t next = (hipStream_t)stream;
+    if (next != h->stream) {
+        // the workspace and counters serve one stream's calls in order: let the old stream's
+        // work finish before another stream's work can reuse them.  Not while either stream is
+        // being captured into a graph: nothing captured runs yet, synchronising is not allowed
+        // then, and the capturing caller orders the streams itself.
+        hipStreamCaptureStatus cs_old = hipStreamCaptureStatusNone, cs_new = hipStreamCaptureStatusNone;
+        const bool q_old = hipStreamIsCapturing(h->stream, &cs_old) == hipSuccess;
+        const bool q_new = hipStreamIsCapturing(next, &cs_new) == hipSuccess;
+        if (q_old && q_new && cs_old == hipStreamCaptureStatusNone && cs_new == hipStreamCaptureStatusNone) {
+            hipError_t e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize (previous stream)");
+        }
+        h->stream = next;
+    }
     return 0;
 }
 
@@ -433,16 +503,16 @@ int mi_llt_status(mi_llt_handle h, int *status)
 {
     if (!valid(h)) return MI_LLT_E_HANDLE;
     if (!status) return -2;
-    *status = 0;
-    hipError_t e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
-    if (!h->sync) return 0;
-    int v = 0;
-    e = hipMemcpy(&v, h->sync, sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && v != 0) e = hipMemset(h->sync, 0, sizeof(int));
-    if (e != hipSuccess) return fail_hip(h, e, "status readback");
-    *status = v;
-    return 0;
+    return read_status(h, status);
+}
+
+int mi_llt_inject_status(mi_llt_handle h, int bits)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    int rc = reserve_sync(h, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(or_int_kernel, dim3(1), dim3(1), 0, h->stream, (int *)h->sync, bits);
+    return check_launch(h, "inject status");
 }
 
 int mi_llt_trace(mi_llt_handle h, int enable)
@@ -519,9 +589,7 @@ int mi_dpotrf(mi_llt_handle h, char uplo, int64_t n, double *dA, int64_t lda, in
     if (info == nullptr) return -6;
     int rc = mi_dpotrf_async(h, uplo, n, dA, lda, h->d_info);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    return e == hipSuccess ? 0 : fail_hip(h, e, "info readback");
+    return read_back(h, info, h->d_info, sizeof(int));
 }
 
 int mi_dpotrs(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double *dA, int64_t lda, double *dB,
@@ -590,6 +658,308 @@ int mi_dposv_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, double *
     return check_launch(h, "posv");
 }
 
+int mi_spotrf_async(mi_llt_handle h, char uplo, int64_t n, float *dA, int64_t lda, int *d_info)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (n < 0) return -3;
+    if (dA == nullptr && n > 0) return -4;
+    if (lda < (n > 1 ? n : 1)) return -5;
+    if (d_info == nullptr) return -6;
+    hipStream_t s = h->stream;
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, d_info, 0);
+    if (n == 0) return check_launch(h, "set_info");
+    Layout lay;  // the 'U' workspace (n * n doubles) holds the n * n floats of the lower copy
+    int rc = prepare(h, n, up == 1, lay);
+    if (rc) return rc;
+    float *L = dA;
+    int64_t ldl = lda;
+    if (up == 1) {
+        L = (float *)lay.tri;
+        ldl = n;
+        mi::launch_upper_to_lower_t<float>(dA, lda, n, L, ldl, s);
+    }
+    rc = factor_lower<float>(h, L, ldl, n, lay.wleaf, d_info, MI_PREC_FP32);
+    if (rc) return rc;
+    if (up == 1) mi::launch_lower_to_upper<float>(L, ldl, n, dA, lda, s);
+    return check_launch(h, "spotrf");
+}
+
+int mi_spotrf(mi_llt_handle h, char uplo, int64_t n, float *dA, int64_t lda, int *info)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    if (info == nullptr) return -6;
+    int rc = mi_spotrf_async(h, uplo, n, dA, lda, h->d_info);
+    if (rc) return rc;
+    return read_back(h, info, h->d_info, sizeof(int));
+}
+
+int mi_spotrs(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const float *dA, int64_t lda, float *dB,
+              int64_t ldb)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (dA == nullptr && n > 0) return -5;
+    if (lda < (n > 1 ? n : 1)) return -6;
+    if (dB == nullptr && n > 0 && nrhs > 0) return -7;
+    if (ldb < (n > 1 ? n : 1)) return -8;
+    if (n == 0 || nrhs == 0) return 0;
+    // [ Wleaf | Winv | lower copy (n * n floats, uplo 'U') | B in FP64 (n * nrhs) ]
+    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
+    const size_t leaf = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double));
+    const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
+    const size_t tri = up == 1 ? round_up((size_t)n * n * sizeof(float)) : 0;
+    const size_t bw = round_up((size_t)n * nrhs * sizeof(double));
+    int rc = reserve(h, leaf + inv + tri + bw);
+    if (rc) return rc;
+    char *p = (char *)h->ws;
+    double *wleaf = (double *)p, *winv = (double *)(p + leaf);
+    float *Lc = (float *)(p + leaf + inv);
+    double *Bd = (double *)(p + leaf + inv + tri);
+    hipStream_t s = h->stream;
+    const float *L = dA;
+    int64_t ldl = lda;
+    if (up == 1) {
+        mi::launch_upper_to_lower_t<float>(dA, lda, n, Lc, n, s);
+        L = Lc;
+        ldl = n;
+    }
+    mi::launch_convert<float, double>(dB, ldb, n, nrhs, Bd, n, s);
+    mi::launch_leaf_inverses<float>(L, ldl, n, wleaf, s);
+    rc = solve_lower<float>(h, L, ldl, n, Bd, n, nrhs, wleaf, winv, nullptr);
+    if (rc) return rc;
+    mi::launch_convert<double, float>(Bd, n, n, nrhs, dB, ldb, s);
+    return check_launch(h, "spotrs");
+}
+
+int mi_dtrsm(mi_llt_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n, double alpha,
+             const double *dA, int64_t lda, double *dB, int64_t ldb)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const bool left = side == 'L' || side == 'l';
+    if (!left && side != 'R' && side != 'r') return -2;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -3;
+    const bool tr = transa == 'T' || transa == 't' || transa == 'C' || transa == 'c';
+    if (!tr && transa != 'N' && transa != 'n') return -4;
+    const bool unit = diag == 'U' || diag == 'u';
+    if (!unit && diag != 'N' && diag != 'n') return -5;
+    if (m < 0) return -6;
+    if (n < 0) return -7;
+    const int64_t k = left ? m : n;  // order of A
+    if (dA == nullptr && k > 0 && alpha != 0.0) return -9;
+    if (lda < (k > 1 ? k : 1)) return -10;
+    if (dB == nullptr && m > 0 && n > 0) return -11;
+    if (ldb < (m > 1 ? m : 1)) return -12;
+    if (m == 0 || n == 0) return 0;
+    hipStream_t s = h->stream;
+    if (alpha == 0.0) {  // BLAS: A is not referenced, B is set to zero
+        mi::launch_scale(dB, ldb, m, n, 0.0, s);
+        return check_launch(h, "trsm");
+    }
+    // Every case is one sweep of the LLT solve on a lower operand L: forward (L Y = C) when the
+    // operator applied to C is lower triangular, backward (L^T Y = C) otherwise.  Left side:
+    // C = B, operator op(A); right side: C = B^T, operator op(A)^T.
+    const bool upper = up == 1;
+    const bool forward = left ? (upper == tr) : (upper != tr);
+    const bool stage = upper || unit;  // the solve reads a lower, non-unit triangle in place
+    const int64_t nrhs = left ? n : m;
+    const int64_t nblk = (k + mi::kNB - 1) / mi::kNB;
+    const size_t leaf = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double));
+    const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
+    const size_t tri = stage ? round_up((size_t)k * k * sizeof(double)) : 0;
+    const size_t ct = left ? 0 : round_up((size_t)k * nrhs * sizeof(double));
+    int rc = reserve(h, leaf + inv + tri + ct);
+    if (rc) return rc;
+    char *p = (char *)h->ws;
+    double *wleaf = (double *)p, *winv = (double *)(p + leaf);
+    double *Lw = (double *)(p + leaf + inv), *Ct = (double *)(p + leaf + inv + tri);
+    const double *L = dA;
+    int64_t ldl = lda;
+    if (stage) {
+        mi::launch_tri_stage(dA, lda, k, upper, unit, Lw, k, s);
+        L = Lw;
+        ldl = k;
+    }
+    double *C = dB;
+    int64_t ldc = ldb;
+    if (left) {
+        if (alpha != 1.0) mi::launch_scale(dB, ldb, m, n, alpha, s);
+    } else {
+        mi::launch_transpose_scale(dB, ldb, m, n, alpha, Ct, k, s);  // C = alpha B^T (n x m)
+        C = Ct;
+        ldc = k;
+    }
+    mi::launch_leaf_inverses<double>(L, ldl, k, wleaf, s);
+    block_inverses<double>(h, L, ldl, k, wleaf, winv, nullptr);
+    if ((rc = solve_only<double>(h, L, ldl, k, C, ldc, nrhs, winv, nullptr, forward ? 1 : 2))) return rc;
+    if (!left) mi::launch_transpose_scale(Ct, k, k, m, 1.0, dB, ldb, s);  // B = C^T
+    return check_launch(h, "trsm");
+}
+
+int mi_dsyrk(mi_llt_handle h, char uplo, char trans, int64_t n, int64_t k, double alpha, const double *dA,
+             int64_t lda, double beta, double *dC, int64_t ldc)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    const bool tr = trans == 'T' || trans == 't' || trans == 'C' || trans == 'c';
+    if (!tr && trans != 'N' && trans != 'n') return -3;
+    if (n < 0) return -4;
+    if (k < 0) return -5;
+    const int64_t arows = tr ? k : n;
+    if (dA == nullptr && n > 0 && k > 0 && alpha != 0.0) return -7;
+    if (lda < (arows > 1 ? arows : 1)) return -8;
+    if (dC == nullptr && n > 0) return -10;
+    if (ldc < (n > 1 ? n : 1)) return -11;
+    if (n == 0) return 0;
+    double *W = nullptr;
+    const int64_t kp = mi::syrk_depth_padded(k);
+    if (alpha != 0.0 && k > 0 && (tr || kp != k)) {
+        int rc = reserve(h, round_up((size_t)n * kp * sizeof(double)));
+        if (rc) return rc;
+        W = (double *)h->ws;
+    }
+    traced(h, h->stream, MI_TRACE_SYRK, (double)n * (double)(n + 1) * (double)k,
+           [&] { mi::launch_syrk(up == 1, tr, n, k, alpha, dA, lda, beta, dC, ldc, W, h->stream); });
+    return check_launch(h, "syrk");
+}
+
+int mi_dllt_rank_update(mi_llt_handle h, char uplo, int64_t n, double *dL, int64_t ldl, const double *dv,
+                        int64_t incv, double sigma, int *d_info)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (n < 0) return -3;
+    if (dL == nullptr && n > 0) return -4;
+    if (ldl < (n > 1 ? n : 1)) return -5;
+    if (dv == nullptr && n > 0) return -6;
+    if (incv < 1) return -7;
+    if (!(sigma == sigma)) return -8;
+    if (d_info == nullptr) return -9;
+    hipStream_t s = h->stream;
+    if (n == 0) {
+        hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, d_info, 0);
+        return check_launch(h, "rank_update");
+    }
+    int rc = reserve_sync(h, 0);
+    if (rc) return rc;
+    // [ coefficients, flags | lower copy for uplo 'U' ]
+    const size_t scr = round_up((size_t)mi::rank_update_scratch_bytes(n));
+    const size_t tri = up == 1 ? round_up((size_t)n * n * sizeof(double)) : 0;
+    if ((rc = reserve(h, scr + tri))) return rc;
+    double *L = dL;
+    int64_t ld = ldl;
+    if (up == 1) {  // the upper factor is the lower factor of the transpose (LLT.h:377-399)
+        L = (double *)((char *)h->ws + scr);
+        ld = n;
+        mi::launch_upper_to_lower(dL, ldl, n, L, ld, s);
+    }
+    mi::launch_rank_update(L, ld, n, dv, incv, sigma, h->ws, (int *)h->sync + 3, d_info, (int *)h->sync, s);
+    if (up == 1) mi::launch_lower_to_upper<double>(L, ld, n, dL, ldl, s);
+    return check_launch(h, "rank_update");
+}
+
+int mi_dpocon(mi_llt_handle h, char uplo, int64_t n, const double *dL, int64_t ldl, double anorm, double *rcond)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (n < 0) return -3;
+    if (dL == nullptr && n > 0) return -4;
+    if (ldl < (n > 1 ? n : 1)) return -5;
+    if (!(anorm >= 0.0)) return -6;
+    if (rcond == nullptr) return -7;
+    // rcond_estimate_helper, ConditionEstimator.h:158-169
+    if (n == 0) {
+        *rcond = std::numeric_limits<double>::infinity();
+        return 0;
+    }
+    if (anorm == 0.0) {
+        *rcond = 0.0;
+        return 0;
+    }
+    if (n == 1) {
+        *rcond = 1.0;
+        return 0;
+    }
+    hipStream_t s = h->stream;
+    // [ Wleaf | Winv | lower copy for uplo 'U' | one vector ]
+    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
+    const size_t leaf = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double));
+    const size_t inv = round_up((size_t)nblk * mi::kInvDoubles * sizeof(double));
+    const size_t tri = up == 1 ? round_up((size_t)n * n * sizeof(double)) : 0;
+    int rc = reserve(h, leaf + inv + tri + round_up((size_t)n * sizeof(double)));
+    if (rc) return rc;
+    char *p = (char *)h->ws;
+    double *wleaf = (double *)p, *winv = (double *)(p + leaf), *dv = (double *)(p + leaf + inv + tri);
+    const double *L = dL;
+    int64_t ld = ldl;
+    if (up == 1) {
+        double *Lc = (double *)(p + leaf + inv);
+        mi::launch_upper_to_lower(dL, ldl, n, Lc, n, s);
+        L = Lc;
+        ld = n;
+    }
+    mi::launch_leaf_inverses<double>(L, ld, n, wleaf, s);
+    block_inverses<double>(h, L, ld, n, wleaf, winv, nullptr);
+    std::vector<double> v((size_t)n), sgn((size_t)n), old_sgn((size_t)n);
+    // v <- A^{-1} v with the factor (the same solve serves dec.solve and dec.adjoint().solve:
+    // A is symmetric, LLT.h:204)
+    auto solve = [&]() -> int {
+        hipError_t e = hipMemcpyAsync(dv, v.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return fail_hip(h, e, "rcond vector");
+        int r = solve_only<double>(h, L, ld, n, dv, n, 1, winv, nullptr);
+        if (r) return r;
+        return read_back(h, v.data(), dv, (size_t)n * sizeof(double));
+    };
+    auto norm1 = [&]() {
+        double a = 0.0;
+        for (int64_t i = 0; i < n; ++i) a += fabs(v[i]);
+        return a;
+    };
+    // rcond_invmatrix_L1_norm_estimate, ConditionEstimator.h:55-141 (real case)
+    for (int64_t i = 0; i < n; ++i) v[i] = 1.0 / (double)n;
+    if ((rc = solve())) return rc;
+    double lower_bound = norm1();
+    double old_lower_bound = lower_bound;
+    int64_t old_j = -1;
+    for (int k = 0; k < 4; ++k) {
+        for (int64_t i = 0; i < n; ++i) sgn[i] = v[i] < 0.0 ? -1.0 : 1.0;
+        if (k > 0 && sgn == old_sgn) break;  // the solution stagnated
+        v = sgn;
+        if ((rc = solve())) return rc;
+        int64_t j = 0;  // first index of max |v_i|
+        for (int64_t i = 1; i < n; ++i)
+            if (fabs(v[i]) > fabs(v[j])) j = i;
+        if (j == old_j) break;
+        std::fill(v.begin(), v.end(), 0.0);  // v = A^{-1} e_j
+        v[j] = 1.0;
+        if ((rc = solve())) return rc;
+        lower_bound = norm1();
+        if (lower_bound <= old_lower_bound) break;  // no increase
+        old_sgn = sgn;
+        old_j = j;
+        old_lower_bound = lower_bound;
+    }
+    // Higham's alternating-sign vector v_i = (-1)^i (1 + i / (n - 1))
+    double alt = 1.0;
+    for (int64_t i = 0; i < n; ++i) {
+        v[i] = alt * (1.0 + (double)i / (double)(n - 1));
+        alt = -alt;
+    }
+    if ((rc = solve())) return rc;
+    const double alternate = (2.0 * norm1()) / (3.0 * (double)n);
+    const double inv_norm = lower_bound > alternate ? lower_bound : alternate;
+    *rcond = inv_norm == 0.0 ? 0.0 : (1.0 / inv_norm) / anorm;
+    return 0;
+}
+
 int mi_dsy_l1norm(mi_llt_handle h, char uplo, int64_t n, const double *dA, int64_t lda, double *d_norm)
 {
     if (!valid(h)) return MI_LLT_E_HANDLE;
@@ -600,8 +970,8 @@ int mi_dsy_l1norm(mi_llt_handle h, char uplo, int64_t n, const double *dA, int64
     if (lda < (n > 1 ? n : 1)) return -5;
     if (d_norm == nullptr) return -6;
     if (n == 0) {
-        hipError_t e = hipMemsetAsync(d_norm, 0, sizeof(double), h->stream);
-        return e == hipSuccess ? 0 : fail_hip(h, e, "l1norm");
+        hipLaunchKernelGGL(set_double_kernel, dim3(1), dim3(1), 0, h->stream, d_norm, 0.0);
+        return check_launch(h, "l1norm");
     }
     int rc = reserve(h, round_up((size_t)n * sizeof(double)) > h->ws_bytes ? round_up((size_t)n * sizeof(double))
                                                                             : h->ws_bytes);

@@ -81,6 +81,13 @@ __device__ __forceinline__ bool factor_failed(const int *info)
     return info != nullptr && __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
+// The handle's status word (timeouts, inconsistent tickets) is set: the kernels of the
+// factorization and the solve then exit without work until the host reads and clears it.
+__device__ __forceinline__ bool device_failed(const int *status)
+{
+    return status != nullptr && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 // Lower-triangular tile enumeration: t -> (ti, tj), ti >= tj, row-major over the lower
 // triangle (t = ti*(ti+1)/2 + tj).
 __device__ __forceinline__ void lower_tile(int t, int &ti, int &tj)

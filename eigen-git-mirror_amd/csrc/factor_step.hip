@@ -63,8 +63,15 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
     // every workgroup draws its ticket, even when it then stops: the host counts them
     if (threadIdx.x == 0) s_role = (int)(atomicAdd(tickets, 1u) - ticket_base);
     __syncthreads();
-    if (factor_failed(info)) return;
+    if (factor_failed(info) || device_failed(status)) return;
     const int t = __builtin_amdgcn_readfirstlane(s_role);  // uniform: every derived address is scalar
+    // A role outside this launch means the counter was not reset with the host's accounting
+    // (never in a correct call sequence): record it and touch nothing, rather than turn the
+    // ticket into an address.
+    if (t < 0 || t >= Gp + nsyrk) {
+        if (threadIdx.x == 0) atomicOr(status, MI_FACTOR_TICKET);
+        return;
+    }
     if (Gp > 0 && (t == 0 || t > nsyrk)) {  // ticket 0: diagonal role; last Gp-1: row roles
         __bf16 *split = Wsplit ? Wsplit + ((kp / kNB) & 3) * slot_elems : nullptr;
         panel_role<S, PREC>(t == 0 ? 0 : t - nsyrk, smem, &s_ok, A, lda, n, kp, Wleaf, info, flag, epoch,

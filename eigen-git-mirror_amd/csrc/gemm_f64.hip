@@ -47,6 +47,37 @@ void launch_upper_to_lower(const double *A, int64_t lda, int64_t n, double *W, i
 }
 
 template <typename T>
+void launch_upper_to_lower_t(const T *A, int64_t lda, int64_t n, T *W, int64_t ldw, hipStream_t s)
+{
+    const unsigned G = (unsigned)((n + 31) / 32);
+    hipLaunchKernelGGL((tri_transpose_kernel<true, T>), dim3(G, G), dim3(256), 0, s, A, lda, n, W, ldw);
+}
+template void launch_upper_to_lower_t<float>(const float *, int64_t, int64_t, float *, int64_t, hipStream_t);
+
+// dst (m x n, ld ldd) = src converted element by element.
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void convert_kernel(const S *__restrict__ src, int64_t lds_, int64_t m, int64_t n,
+                                                      D *__restrict__ dst, int64_t ldd)
+{
+    const int64_t total = m * n;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e % m, j = e / m;
+        dst[i + j * ldd] = (D)src[i + j * lds_];
+    }
+}
+
+template <typename S, typename D>
+void launch_convert(const S *src, int64_t lds_, int64_t m, int64_t n, D *dst, int64_t ldd, hipStream_t s)
+{
+    const int64_t total = m * n;
+    if (total == 0) return;
+    const unsigned g = (unsigned)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+    hipLaunchKernelGGL((convert_kernel<S, D>), dim3(g), dim3(256), 0, s, src, lds_, m, n, dst, ldd);
+}
+template void launch_convert<float, double>(const float *, int64_t, int64_t, int64_t, double *, int64_t, hipStream_t);
+template void launch_convert<double, float>(const double *, int64_t, int64_t, int64_t, float *, int64_t, hipStream_t);
+
+template <typename T>
 void launch_lower_to_upper(const T *W, int64_t ldw, int64_t n, T *A, int64_t lda, hipStream_t s)
 {
     const unsigned G = (unsigned)((n + 31) / 32);

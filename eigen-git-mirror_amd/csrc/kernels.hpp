@@ -28,6 +28,7 @@ constexpr int64_t kInvM2 = 4 * 128 * 128, kInvN2 = 5 * 128 * 128;
 // Device status bits (a word in the handle's persistent device memory).
 constexpr int MI_SOLVE_TIMEOUT = 1;
 constexpr int MI_PANEL_TIMEOUT = 2;
+constexpr int MI_FACTOR_TICKET = 4;  // a factorization workgroup drew a ticket outside its launch
 
 // Launch k of the factorization: panel of block column kp (or none, kp < 0) + the trailing
 // update by panels k-1, k of the block columns k+2, k+4, ... (none for k < 0).  Writes the grid
@@ -98,6 +99,33 @@ template <typename T>
 void launch_vbatched(int64_t batch, const int *d_n, T *const *d_A, const int *d_lda, const int *d_nrhs,
                      T *const *d_B, const int *d_ldb, int *d_info, int *order, int *counter, T *scratch, int grid,
                      int group, int lpt, hipStream_t s);
+
+// Level-3 pieces of the BLAS route (blas3.hip).
+// L (lower, ld ldl) = the lower triangle of A, or the upper one transposed; unit: diagonal 1.
+void launch_tri_stage(const double *A, int64_t lda, int64_t k, bool upper, bool unit, double *L, int64_t ldl,
+                      hipStream_t s);
+// D (n x m) = alpha S^T for S m x n; alpha == 0 writes zeros without reading S.
+void launch_transpose_scale(const double *S, int64_t lds_, int64_t m, int64_t n, double alpha, double *D,
+                            int64_t ldd, hipStream_t s);
+// B (m x n) *= alpha; alpha == 0 writes zeros without reading B.
+void launch_scale(double *B, int64_t ldb, int64_t m, int64_t n, double alpha, hipStream_t s);
+// C = alpha op(A) op(A)^T + beta C on the `upper` or lower triangle (op(A) = A, n x k, or
+// A^T, A k x n, when trans); W: n * syrk_depth_padded(k) doubles of scratch (used when trans
+// or when k is not a multiple of 16).
+int64_t syrk_depth_padded(int64_t k);
+void launch_syrk(bool upper, bool trans, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                 double beta, double *C, int64_t ldc, double *W, hipStream_t s);
+// Precision conversion of an m x n column-major block (S -> D).
+template <typename S, typename D>
+void launch_convert(const S *src, int64_t lds_, int64_t m, int64_t n, D *dst, int64_t ldd, hipStream_t s);
+template <typename T>
+void launch_upper_to_lower_t(const T *A, int64_t lda, int64_t n, T *W, int64_t ldw, hipStream_t s);
+
+// Rank-one update / downdate of a lower factor (rank_update.hip).  scratch:
+// rank_update_scratch_bytes(n) bytes; ticket, info: device ints (reset by the launch).
+int64_t rank_update_scratch_bytes(int64_t n);
+void launch_rank_update(double *L, int64_t ldl, int64_t n, const double *v, int64_t incv, double sigma, void *scratch,
+                        int *ticket, int *info, int *status, hipStream_t s);
 
 // Reverse mode (adjoint.hip).
 // P (lower) = tril(L^T G) for lower L, G, from Lt = tril(L)^T and Gt = tril(G)^T; Wp: n*n

@@ -68,7 +68,8 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
                                                           const double *__restrict__ Winv, double *Ypk, int *ticket,
                                                           const int *info, int *status)
 {
-    if (factor_failed(info)) return;  // uniform: info is final before this kernel starts
+    // uniform: info and the status word are final before this kernel starts
+    if (factor_failed(info) || device_failed(status)) return;
     // Ys[c + SC*k + 16 (k / 32)]: the published block being consumed, one padding row per
     // 32 k, so that lanes reading k = 32 q + s (q = lane / 16, the permuted order below) hit
     // distinct banks

@@ -30,7 +30,15 @@ _INT = ctypes.c_int
 
 
 class MiLLTError(RuntimeError):
-    """A status code other than 0 from the C ABI."""
+    """A status code other than 0 from the C ABI (``code`` holds it)."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
+
+
+E_DEVICE = -1005  # MI_LLT_E_DEVICE: the device reported a failure of queued work
+STATUS_SOLVE_TIMEOUT, STATUS_PANEL_TIMEOUT, STATUS_FACTOR_TICKET = 1, 2, 4
 
 
 def lib() -> ctypes.CDLL:
@@ -57,6 +65,7 @@ def lib() -> ctypes.CDLL:
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]
         L.mi_llt_trace_reset.argtypes = [_VP]
         L.mi_llt_status.argtypes = [_VP, ctypes.POINTER(_INT)]
+        L.mi_llt_inject_status.argtypes = [_VP, _INT]
         L.mi_dposv_vbatched.argtypes = [_VP, ctypes.c_char, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
         L.mi_sposv_vbatched.argtypes = [_VP, ctypes.c_char, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
         L.mi_llt_set_vbatched_config.argtypes = [_VP, _INT, _INT, _INT]
@@ -67,6 +76,28 @@ def lib() -> ctypes.CDLL:
         L.mi_dposv_mixed.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _INT,
                                      ctypes.c_double, _INT, ctypes.POINTER(_INT), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(_INT)]
+        L.mi_spotrf.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, ctypes.POINTER(_INT)]
+        L.mi_spotrf_async.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, _VP]
+        L.mi_spotrs.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64]
+        L.mi_dtrsm.argtypes = [_VP, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, _I64, _I64,
+                               ctypes.c_double, _VP, _I64, _VP, _I64]
+        L.mi_dsyrk.argtypes = [_VP, ctypes.c_char, ctypes.c_char, _I64, _I64, ctypes.c_double, _VP, _I64,
+                               ctypes.c_double, _VP, _I64]
+        L.mi_dpocon.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.mi_dllt_rank_update.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, _VP, _I64, ctypes.c_double, _VP]
+        L.LAPACKE_spotrf.argtypes = [_INT, ctypes.c_char, _INT, _VP, _INT]
+        L.LAPACKE_spotrs.argtypes = [_INT, ctypes.c_char, _INT, _INT, _VP, _INT, _VP, _INT]
+        _PI = ctypes.POINTER(_INT)
+        _PD = ctypes.POINTER(ctypes.c_double)
+        L.spotrf_.argtypes = [ctypes.c_char_p, _PI, _VP, _PI, _PI]
+        L.spotrs_.argtypes = [ctypes.c_char_p, _PI, _PI, _VP, _PI, _VP, _PI, _PI]
+        L.dtrsm_.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _PI, _PI, _PD,
+                             _VP, _PI, _VP, _PI]
+        L.dsyrk_.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _PI, _PI, _PD, _VP, _PI, _PD, _VP, _PI]
+        L.spotrf_.restype = None
+        L.spotrs_.restype = None
+        L.dtrsm_.restype = _INT
+        L.dsyrk_.restype = _INT
         L.LAPACKE_dpotrf.argtypes = [_INT, ctypes.c_char, _INT, _VP, _INT]
         L.LAPACKE_dpotrs.argtypes = [_INT, ctypes.c_char, _INT, _INT, _VP, _INT, _VP, _INT]
         L.dpotrf_.argtypes = [ctypes.c_char_p, ctypes.POINTER(_INT), _VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]
@@ -74,9 +105,11 @@ def lib() -> ctypes.CDLL:
                               ctypes.POINTER(_INT), _VP, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]
         for name in ("mi_llt_create", "mi_llt_destroy", "mi_llt_set_stream", "mi_dpotrf", "mi_dpotrf_async",
                      "mi_dpotrs", "mi_dposv_async", "mi_dsy_l1norm", "LAPACKE_dpotrf", "LAPACKE_dpotrs",
-                     "mi_llt_trace", "mi_llt_trace_read", "mi_llt_trace_reset", "mi_llt_status",
+                     "mi_llt_trace", "mi_llt_trace_read", "mi_llt_trace_reset", "mi_llt_status", "mi_llt_inject_status",
                      "mi_dposv_mixed", "mi_dposv_vbatched", "mi_sposv_vbatched", "mi_llt_set_vbatched_config",
-                     "mi_llt_get_vbatched_config", "mi_dpotrf_adjoint", "mi_dpotrs_adjoint"):
+                     "mi_llt_get_vbatched_config", "mi_dpotrf_adjoint", "mi_dpotrs_adjoint", "mi_spotrf",
+                     "mi_spotrf_async", "mi_spotrs", "mi_dtrsm", "mi_dsyrk", "mi_dpocon", "mi_dllt_rank_update",
+                     "LAPACKE_spotrf", "LAPACKE_spotrs"):
             getattr(L, name).restype = _INT
         L.dpotrf_.restype = None
         L.dpotrs_.restype = None
@@ -126,7 +159,7 @@ class Handle:
     def check(self, rc: int, what: str):
         if rc != 0:
             msg = lib().mi_llt_last_error(self._h).decode(errors="replace")
-            raise MiLLTError(f"{what} returned {rc}: {msg}")
+            raise MiLLTError(f"{what} returned {rc}: {msg}", rc)
 
     def workspace_bytes(self) -> int:
         return int(lib().mi_llt_workspace_bytes(self._h))
@@ -136,6 +169,10 @@ class Handle:
         v = ctypes.c_int(0)
         self.check(lib().mi_llt_status(self._h, ctypes.byref(v)), "mi_llt_status")
         return int(v.value)
+
+    def inject_status(self, bits: int):
+        """Sets device status bits on the handle's stream (tests of the failure path)."""
+        self.check(lib().mi_llt_inject_status(self._h, int(bits)), "mi_llt_inject_status")
 
     TRACE_KINDS = {"factor_step": 0, "solve": 3, "trtri": 4, "residual": 5, "batched": 6}
 
@@ -214,6 +251,69 @@ def posv_async_(a: torch.Tensor, b: torch.Tensor, d_info: torch.Tensor, uplo: st
                                  _ld(b), _VP(d_info.data_ptr())), "mi_dposv_async")
 
 
+def spotrf_(a: torch.Tensor, uplo: str = "L", handle: Handle | None = None) -> int:
+    """In-place Cholesky of a column-major float32 ``a`` (``mi_spotrf``); returns LAPACK info."""
+    h = handle.bind_stream() if handle else default_handle()
+    if a.dtype != torch.float32:
+        raise ValueError("spotrf_ needs float32")
+    info = ctypes.c_int(0)
+    h.check(lib().mi_spotrf(h.ptr, _uplo(uplo), int(a.shape[0]), _VP(a.data_ptr()), _ld(a), ctypes.byref(info)),
+            "mi_spotrf")
+    return int(info.value)
+
+
+def spotrs_(l: torch.Tensor, b: torch.Tensor, uplo: str = "L", handle: Handle | None = None):
+    """In-place float32 solve ``b <- A^{-1} b`` with the factor from :func:`spotrf_`."""
+    h = handle.bind_stream() if handle else default_handle()
+    if l.dtype != torch.float32 or b.dtype != torch.float32:
+        raise ValueError("spotrs_ needs float32")
+    h.check(lib().mi_spotrs(h.ptr, _uplo(uplo), int(l.shape[0]), int(b.shape[1]), _VP(l.data_ptr()), _ld(l),
+                            _VP(b.data_ptr()), _ld(b)), "mi_spotrs")
+
+
+def trsm_(a: torch.Tensor, b: torch.Tensor, side: str = "L", uplo: str = "L", trans: str = "N", diag: str = "N",
+          alpha: float = 1.0, handle: Handle | None = None):
+    """``b <- alpha op(a)^{-1} b`` (side 'L') or ``alpha b op(a)^{-1}`` (side 'R'), float64 (``mi_dtrsm``)."""
+    h = handle.bind_stream() if handle else default_handle()
+    m, n = int(b.shape[0]), int(b.shape[1])
+    h.check(lib().mi_dtrsm(h.ptr, side.encode(), uplo.encode(), trans.encode(), diag.encode(), m, n, float(alpha),
+                           _VP(a.data_ptr()), _ld(a), _VP(b.data_ptr()), _ld(b)), "mi_dtrsm")
+
+
+def syrk_(a: torch.Tensor, c: torch.Tensor, uplo: str = "L", trans: str = "N", alpha: float = 1.0,
+          beta: float = 1.0, handle: Handle | None = None):
+    """``c <- alpha op(a) op(a)^T + beta c`` on the ``uplo`` triangle, float64 (``mi_dsyrk``)."""
+    h = handle.bind_stream() if handle else default_handle()
+    n = int(c.shape[0])
+    k = int(a.shape[1]) if trans in ("N", "n") else int(a.shape[0])
+    h.check(lib().mi_dsyrk(h.ptr, uplo.encode(), trans.encode(), n, k, float(alpha), _VP(a.data_ptr()), _ld(a),
+                           float(beta), _VP(c.data_ptr()), _ld(c)), "mi_dsyrk")
+
+
+def pocon(l: torch.Tensor, anorm: float, uplo: str = "L", handle: Handle | None = None) -> float:
+    """Reciprocal condition estimate from the factor in ``l`` and ``anorm`` = ||A||_1 (``mi_dpocon``)."""
+    h = handle.bind_stream() if handle else default_handle()
+    out = ctypes.c_double(0.0)
+    h.check(lib().mi_dpocon(h.ptr, _uplo(uplo), int(l.shape[0]), _VP(l.data_ptr()), _ld(l), float(anorm),
+                            ctypes.byref(out)), "mi_dpocon")
+    return float(out.value)
+
+
+def rank_update_(l: torch.Tensor, v: torch.Tensor, sigma: float = 1.0, uplo: str = "L",
+                 handle: Handle | None = None) -> int:
+    """The factor of ``L L^T + sigma v v^T`` in place of ``l`` (``mi_dllt_rank_update``); returns
+    LAPACK-style info (0, or j+1 when the result is not positive definite at column j)."""
+    h = handle.bind_stream() if handle else default_handle()
+    n = int(l.shape[0])
+    if v.numel() != n or v.dtype != torch.float64 or v.device != l.device:
+        raise ValueError("rank_update_ needs a float64 vector of n entries on the factor's device")
+    inc = 1 if v.dim() == 1 else (int(v.stride(0)) if v.shape[0] == n else int(v.stride(1)))
+    d_info = torch.zeros(1, dtype=torch.int32, device=l.device)
+    h.check(lib().mi_dllt_rank_update(h.ptr, _uplo(uplo), n, _VP(l.data_ptr()), _ld(l), _VP(v.data_ptr()),
+                                      max(inc, 1), float(sigma), _VP(d_info.data_ptr())), "mi_dllt_rank_update")
+    return int(d_info.item())
+
+
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16x3": 3, "bf16x6": 4}
 
 
@@ -251,4 +351,5 @@ def sy_l1norm(a: torch.Tensor, uplo: str = "L", handle: Handle | None = None) ->
 from .llt import LLT, ComputationInfo  # noqa: E402
 
 __all__ = ["lib", "colmajor", "Handle", "default_handle", "potrf_", "potrf_async_", "potrs_", "posv_async_",
-           "posv_mixed", "PREC", "sy_l1norm", "LLT", "ComputationInfo", "MiLLTError", "LIB_PATH"]
+           "posv_mixed", "PREC", "sy_l1norm", "spotrf_", "spotrs_", "trsm_", "syrk_", "pocon", "rank_update_",
+           "LLT", "ComputationInfo", "MiLLTError", "LIB_PATH", "E_DEVICE"]

@@ -22,13 +22,20 @@ extern "C" {
 
 /* Eigen/src/misc/lapacke.h:2823 -- used by LLT_LAPACKE.h:59-61 */
 int LAPACKE_dpotrf(int matrix_layout, char uplo, int n, double *a, int lda);
+/* Eigen/src/misc/lapacke.h:2821 -- LLT<MatrixXf> through LLT_LAPACKE.h:90 (mi_spotrf) */
+int LAPACKE_spotrf(int matrix_layout, char uplo, int n, float *a, int lda);
 /* Eigen/src/misc/lapacke.h:2842 (dpotrs) -- the solve of lapack/cholesky.cpp:41-72 */
 int LAPACKE_dpotrs(int matrix_layout, char uplo, int n, int nrhs, const double *a, int lda, double *b, int ldb);
+/* Eigen/src/misc/lapacke.h:2839 (spotrs) -- the float solve (mi_spotrs) */
+int LAPACKE_spotrs(int matrix_layout, char uplo, int n, int nrhs, const float *a, int lda, float *b, int ldb);
 
-/* Fortran ABI: lapack/cholesky.cpp:14-36 (potrf) and :41-72 (potrs); names via
- * EIGEN_LAPACK_FUNC -> prefix + name + '_' (lapack/lapack_common.h:16-18). */
+/* Fortran ABI: lapack/cholesky.cpp:14-36 (potrf) and :41-72 (potrs), instantiated for d and s;
+ * names via EIGEN_LAPACK_FUNC -> prefix + name + '_' (lapack/lapack_common.h:16-18). */
 void dpotrf_(const char *uplo, const int *n, double *a, const int *lda, int *info);
+void spotrf_(const char *uplo, const int *n, float *a, const int *lda, int *info);
 void dpotrs_(const char *uplo, const int *n, const int *nrhs, const double *a, const int *lda, double *b,
+             const int *ldb, int *info);
+void spotrs_(const char *uplo, const int *n, const int *nrhs, const float *a, const int *lda, float *b,
              const int *ldb, int *info);
 
 #ifdef __cplusplus

@@ -37,6 +37,10 @@ typedef struct mi_llt_context *mi_llt_handle;
 #define MI_LLT_E_ALLOC (-1002)    /* device workspace allocation failed */
 #define MI_LLT_E_HANDLE (-1003)   /* null or destroyed handle */
 #define MI_LLT_E_UNSUPPORTED (-1004)
+#define MI_LLT_E_DEVICE (-1005)   /* the device reported a failure of the queued work (a status bit of
+                                     mi_llt_status: a timeout or a ticket outside its launch); the
+                                     results of that work are invalid.  Synchronous entry points read
+                                     and clear the status word and return this code. */
 
 /* Trailing-update precision for mi_dposv_mixed. */
 typedef enum {
@@ -51,7 +55,10 @@ typedef enum {
 /* ---- handle ------------------------------------------------------------------------ */
 int mi_llt_create(mi_llt_handle *h);
 int mi_llt_destroy(mi_llt_handle h);
-/* Use `stream` (a hipStream_t) for all later work; NULL selects the null stream. */
+/* Use `stream` (a hipStream_t) for all later work; NULL selects the null stream.  When the
+   stream changes, the call first waits for the work queued on the previous one (unless that
+   stream is being captured into a graph), since the handle's workspace and counters serve one
+   stream's calls in order. */
 int mi_llt_set_stream(mi_llt_handle h, void *stream);
 /* Text of the last runtime failure on this handle ("" if none). */
 const char *mi_llt_last_error(mi_llt_handle h);
@@ -60,11 +67,20 @@ const char *mi_llt_last_error(mi_llt_handle h);
 int64_t mi_llt_workspace_bytes(mi_llt_handle h);
 /* Waits for the handle's stream, then returns (and clears) the device status bits of the
  * asynchronous calls since the last query: 0, or a combination of MI_LLT_STATUS_SOLVE_TIMEOUT
- * (a solve sweep gave up waiting for a block) and MI_LLT_STATUS_PANEL_TIMEOUT (a panel solve
- * gave up waiting for its diagonal block); the affected result is then invalid. */
+ * (a solve sweep gave up waiting for a block), MI_LLT_STATUS_PANEL_TIMEOUT (a panel solve gave
+ * up waiting for its diagonal block) and MI_LLT_STATUS_FACTOR_TICKET (a factorization workgroup
+ * drew a ticket outside its launch and stopped); the affected result is then invalid.  While a
+ * bit is set, the factorization and solve kernels queued on the handle exit without work, so
+ * an asynchronous caller checks the status before trusting later results.  Synchronous entry
+ * points (mi_dpotrf, mi_dposv_mixed, the host shims) read and clear it themselves and return
+ * MI_LLT_E_DEVICE when it is set. */
 #define MI_LLT_STATUS_SOLVE_TIMEOUT 1
 #define MI_LLT_STATUS_PANEL_TIMEOUT 2
+#define MI_LLT_STATUS_FACTOR_TICKET 4
 int mi_llt_status(mi_llt_handle h, int *status);
+/* Sets `bits` in the handle's status word on its stream, as a device failure would (no
+ * reference counterpart: for tests of the failure path, which must not make a kernel fail). */
+int mi_llt_inject_status(mi_llt_handle h, int bits);
 
 /* ---- measurement (no reference counterpart: Eigen times with bench/BenchTimer.h) -------
  * When tracing is on, each launch of the kinds below is bracketed by HIP events on the
@@ -107,10 +123,56 @@ int mi_dpotrs(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double 
 int mi_dposv_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, double *dA, int64_t lda,
                    double *dB, int64_t ldb, int *d_info);
 
+/* ---- single precision (LLT<MatrixXf>) -------------------------------------------------
+ * The same factorization and solve for float storage, the plug of LLT_LAPACKE.h:90
+ * (EIGEN_LAPACKE_LLT(float, float, s) -> LAPACKE_spotrf) and of spotrf_/spotrs_
+ * (lapack/cholesky.cpp:14-72, the s-prefixed instance).  FP32 storage; the trailing updates
+ * run on FP32 MFMA with FP32 accumulation, the diagonal-block factor and the panel solve in
+ * FP64 rounded to FP32, the solve in FP64 on the FP32 factor with the result rounded to FP32.
+ * Same argument checks, info convention and uplo semantics as the FP64 entry points. */
+int mi_spotrf(mi_llt_handle h, char uplo, int64_t n, float *dA, int64_t lda, int *info);
+int mi_spotrf_async(mi_llt_handle h, char uplo, int64_t n, float *dA, int64_t lda, int *d_info);
+int mi_spotrs(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const float *dA, int64_t lda, float *dB,
+              int64_t ldb);
+
 /* m_l1_norm of LLT::compute (LLT.h:442-453): max column sum of |.| of the symmetric matrix
  * whose `uplo` triangle is stored at dA.  Writes one double to the device pointer d_norm. */
 int mi_dsy_l1norm(mi_llt_handle h, char uplo, int64_t n, const double *dA, int64_t lda,
                   double *d_norm);
+
+/* ---- rest of the LLT surface ----------------------------------------------------------
+ * mi_dpocon: reciprocal condition number estimate of A from its factor (as left by mi_dpotrf
+ *   in the `uplo` triangle of dL) and anorm = ||A||_1 (mi_dsy_l1norm; LLT keeps it as
+ *   m_l1_norm): LLT::rcond (LLT.h:166-171) = rcond_estimate_helper (ConditionEstimator.h:
+ *   158-169) over rcond_invmatrix_L1_norm_estimate (:55-141, Hager's method with Higham's
+ *   alternating-sign vector, at most 10 solves with the factor, run on the device).  n == 0
+ *   gives +inf, anorm == 0 gives 0, n == 1 gives 1.  Host `rcond`; waits for the stream.
+ * mi_dllt_rank_update: the factor of L L^T + sigma v v^T in place of L (`uplo` triangle of
+ *   dL, n x n; v: n doubles with stride incv >= 1): LLT::rankUpdate (LLT.h:467-480) ->
+ *   llt_rank_update_lower (LLT.h:239-307), Givens rotations for sigma > 0 and the update of
+ *   LLT.h:271-296 otherwise.  d_info (device int): 0, or j+1 when the updated matrix is not
+ *   positive definite at column j (sigma <= 0; columns before j are updated, the rest of the
+ *   factor is left as it was, as the reference leaves it).  Asynchronous. */
+int mi_dpocon(mi_llt_handle h, char uplo, int64_t n, const double *dL, int64_t ldl, double anorm, double *rcond);
+int mi_dllt_rank_update(mi_llt_handle h, char uplo, int64_t n, double *dL, int64_t ldl, const double *dv,
+                        int64_t incv, double sigma, int *d_info);
+
+/* ---- level-3 BLAS on the device (the EIGEN_USE_BLAS route of this path) ------------------
+ * mi_dtrsm: B <- alpha op(A)^{-1} B (side 'L', A m x m) or alpha B op(A)^{-1} (side 'R', A
+ *   n x n); A triangular in its `uplo` triangle (the other is not read), op(A) = A ('N') or A^T
+ *   ('T' or 'C'), diag 'U' unit (the diagonal is not read) or 'N'; B m x n.  Replaces
+ *   triangular_solve_matrix as specialised for ?trsm_ (TriangularSolverMatrix_BLAS.h:41-161;
+ *   LLT::solve's sweeps and the panel solve of LLT.h:364), semantics of blas/level3_impl.h:78-178.
+ * mi_dsyrk: C <- alpha op(A) op(A)^T + beta C on the `uplo` triangle of C (n x n; the other
+ *   triangle is not touched), op(A) = A (n x k, trans 'N') or A^T (A k x n, trans 'T' or 'C');
+ *   beta == 0 does not read C.  Replaces general_matrix_matrix_rankupdate as specialised for
+ *   ?syrk_ (GeneralMatrixMatrixTriangular_BLAS.h:47-134; the trailing update of LLT.h:365),
+ *   semantics of blas/level3_impl.h:357-420.
+ * Both asynchronous on the handle's stream. */
+int mi_dtrsm(mi_llt_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n, double alpha,
+             const double *dA, int64_t lda, double *dB, int64_t ldb);
+int mi_dsyrk(mi_llt_handle h, char uplo, char trans, int64_t n, int64_t k, double alpha, const double *dA,
+             int64_t lda, double beta, double *dC, int64_t ldc);
 
 /* ---- mixed precision + FP64 iterative refinement --------------------------------------
  * No reference counterpart: LLT computes in one precision.  Solves A X = B for symmetric

@@ -9,6 +9,8 @@
 //   Eigen::LLT<MatrixXd>::compute / solve     Eigen/src/Cholesky/LLT.h:430-460, 483-498
 //   internal::llt_inplace<double,Lower>::blocked   Eigen/src/Cholesky/LLT.h:338-368
 //   LLT::rcond                                 Eigen/src/Cholesky/LLT.h:166-171
+//   llt_inplace::rankUpdate                    Eigen/src/Cholesky/LLT.h:239-307, 401-404
+//   llt_inplace<float,Lower>::blocked          Eigen/src/Cholesky/LLT.h:338-368 (float)
 #include <Eigen/Cholesky>
 #include <chrono>
 #include <cstdint>
@@ -74,6 +76,47 @@ int eigen_ref_llt_factor_solve(int64_t n, const double *a, int64_t lda, double *
     if (rcond) *rcond = (llt.info() == Eigen::Success && n > 0) ? llt.rcond() : 0.0;
     (void)l1_norm;
     return llt.info() == Eigen::Success ? 0 : 1;
+}
+
+// In-place blocked factor of a float matrix (LLT<MatrixXf>'s llt_inplace<float,Lower>).
+int64_t eigen_ref_llt_inplace_f32(int64_t n, float *a, int64_t lda)
+{
+    typedef Eigen::Map<Eigen::MatrixXf, 0, Eigen::OuterStride<> > MapXf;
+    MapXf A(a, n, n, Eigen::OuterStride<>(lda));
+    return (int64_t)Eigen::internal::llt_inplace<float, Eigen::Lower>::blocked(A);
+}
+
+// Rank-one update of a lower factor in place: llt_inplace<double,Lower>::rankUpdate
+// (LLT.h:239-307, 401-404).  Returns Eigen's index (-1 = success, else the failing column).
+int64_t eigen_ref_llt_rank_update(int64_t n, double *l, int64_t ldl, const double *v, double sigma)
+{
+    MapXd L(l, n, n, Eigen::OuterStride<>(ldl));
+    Eigen::Map<const Eigen::VectorXd> V(v, n);
+    return (int64_t)Eigen::internal::llt_inplace<double, Eigen::Lower>::rankUpdate(L, V, sigma);
+}
+
+// LLT<MatrixXd>(A).rcond() (LLT.h:166-171); -1 when the factorization fails.
+double eigen_ref_llt_rcond(int64_t n, const double *a, int64_t lda)
+{
+    CMapXd A(a, n, n, Eigen::OuterStride<>(lda));
+    Eigen::LLT<MatrixXd> llt(A);
+    if (llt.info() != Eigen::Success) return -1.0;
+    return llt.rcond();
+}
+
+// Gradient of the Gaussian-process log marginal likelihood with respect to K, from Eigen's
+// LLT on the CPU: G = 1/2 (alpha alpha^T - K^{-1}), alpha = K^{-1} y (K symmetric positive
+// definite, lower triangle read).  Returns 0, or 1 when the factorization fails.
+int eigen_ref_gp_grad(int64_t n, const double *k, int64_t ldk, const double *y, double *g, int64_t ldg)
+{
+    CMapXd K(k, n, n, Eigen::OuterStride<>(ldk));
+    Eigen::LLT<MatrixXd> llt(K);
+    if (llt.info() != Eigen::Success) return 1;
+    Eigen::VectorXd alpha = llt.solve(Eigen::Map<const Eigen::VectorXd>(y, n));
+    MatrixXd kinv = llt.solve(MatrixXd::Identity(n, n));
+    MapXd G(g, n, n, Eigen::OuterStride<>(ldg));
+    G = 0.5 * (alpha * alpha.transpose() - kinv);
+    return 0;
 }
 
 // Timed factor+solve only (bench CPU leg): copies are outside the timed region.

@@ -220,3 +220,254 @@ double oracle_solve_berr(const double *a, idx_t lda, idx_t n, const double *x, i
     }
     return (double)worst;
 }
+
+/* ------------------------------------------------------------------------------------
+ * Single precision: llt_inplace<float,Lower>::blocked / unblocked (LLT.h:312-368), the same
+ * algorithm in FP32 arithmetic (LLT<MatrixXf>, the LAPACKE_spotrf plug of LLT_LAPACKE.h:90). */
+idx_t oracle_llt_unblocked_f32(float *a, idx_t n, idx_t ld)
+{
+    for (idx_t k = 0; k < n; ++k) {
+        idx_t rs = n - k - 1;
+        float x = A_(k, k);
+        if (k > 0) {
+            float s = 0.0f;
+            for (idx_t j = 0; j < k; ++j) s += A_(k, j) * A_(k, j);
+            x -= s;
+        }
+        if (x <= 0.0f) return k;
+        x = sqrtf(x);
+        A_(k, k) = x;
+        if (k > 0 && rs > 0) {
+            for (idx_t j = 0; j < k; ++j) {
+                float akj = A_(k, j);
+                for (idx_t i = k + 1; i < n; ++i) A_(i, k) -= A_(i, j) * akj;
+            }
+        }
+        if (rs > 0)
+            for (idx_t i = k + 1; i < n; ++i) A_(i, k) /= x;
+    }
+    return -1;
+}
+
+idx_t oracle_llt_blocked_f32(float *a, idx_t n, idx_t ld)
+{
+    if (n < 32) return oracle_llt_unblocked_f32(a, n, ld);
+    idx_t block = oracle_llt_block_size(n);
+    for (idx_t k = 0; k < n; k += block) {
+        idx_t bs = (block < n - k) ? block : n - k;
+        idx_t rs = n - k - bs;
+        float *a11 = &A_(k, k), *a21 = &A_(k + bs, k), *a22 = &A_(k + bs, k + bs);
+        idx_t ret = oracle_llt_unblocked_f32(a11, bs, ld);
+        if (ret >= 0) return k + ret;
+        for (idx_t j = 0; j < bs && rs > 0; ++j) { /* A21 <- A21 L11^{-T} (LLT.h:364) */
+            for (idx_t q = 0; q < j; ++q) {
+                float ljq = a11[j + q * ld];
+                for (idx_t i = 0; i < rs; ++i) a21[i + j * ld] -= a21[i + q * ld] * ljq;
+            }
+            for (idx_t i = 0; i < rs; ++i) a21[i + j * ld] /= a11[j + j * ld];
+        }
+        for (idx_t j = 0; j < rs; ++j) /* A22 -= A21 A21^T, lower (LLT.h:365) */
+            for (idx_t q = 0; q < bs; ++q) {
+                float pjq = a21[j + q * ld];
+                for (idx_t i = j; i < rs; ++i) a22[i + j * ld] -= a21[i + q * ld] * pjq;
+            }
+    }
+    return -1;
+}
+
+/* ------------------------------------------------------------------------------------
+ * Eigen/src/Cholesky/LLT.h:239-307 -- llt_rank_update_lower: the factor of L L^T + sigma v v^T
+ * in place of the lower factor l.  sigma > 0: Givens rotations, makeGivens(L_ii, -temp_i,
+ * &L_ii) (Eigen/src/Jacobi/Jacobi.h:231-266) applied to (L(i+1:, i), temp(i+1:)) as
+ * x' = c x + s y, y' = -s x + c y (Jacobi.h:331-332); sigma <= 0: the update of LLT.h:271-296.
+ * `work`: n doubles.  Returns -1, or the column j at which x <= 0 (sigma <= 0). */
+idx_t oracle_llt_rank_update(double *a, idx_t n, idx_t ld, const double *v, double sigma, double *work)
+{
+    double *temp = work;
+    if (sigma > 0.0) {
+        double sq = sqrt(sigma);
+        for (idx_t i = 0; i < n; ++i) temp[i] = sq * v[i];
+        for (idx_t i = 0; i < n; ++i) {
+            double p = A_(i, i), q = -temp[i], c, s, r;
+            if (q == 0.0) {
+                c = p < 0.0 ? -1.0 : 1.0;
+                s = 0.0;
+                r = fabs(p);
+            } else if (p == 0.0) {
+                c = 0.0;
+                s = q < 0.0 ? 1.0 : -1.0;
+                r = fabs(q);
+            } else if (fabs(p) > fabs(q)) {
+                double t = q / p, u = sqrt(1.0 + t * t);
+                if (p < 0.0) u = -u;
+                c = 1.0 / u;
+                s = -t * c;
+                r = p * u;
+            } else {
+                double t = p / q, u = sqrt(1.0 + t * t);
+                if (q < 0.0) u = -u;
+                s = -1.0 / u;
+                c = -t * s;
+                r = q * u;
+            }
+            A_(i, i) = r;
+            for (idx_t k = i + 1; k < n; ++k) {
+                double xi = A_(k, i), yi = temp[k];
+                A_(k, i) = c * xi + s * yi;
+                temp[k] = -s * xi + c * yi;
+            }
+        }
+        return -1;
+    }
+    for (idx_t i = 0; i < n; ++i) temp[i] = v[i];
+    double beta = 1.0;
+    for (idx_t j = 0; j < n; ++j) {
+        double ljj = A_(j, j), dj = ljj * ljj, wj = temp[j], swj2 = sigma * wj * wj;
+        double gamma = dj * beta + swj2;
+        double x = dj + swj2 / beta;
+        if (x <= 0.0) return j;
+        double nljj = sqrt(x);
+        A_(j, j) = nljj;
+        beta += swj2 / dj;
+        for (idx_t k = j + 1; k < n; ++k) temp[k] -= (wj / ljj) * A_(k, j);
+        if (gamma != 0.0)
+            for (idx_t k = j + 1; k < n; ++k)
+                A_(k, j) = (nljj / ljj) * A_(k, j) + (nljj * sigma * wj / gamma) * temp[k];
+    }
+    return -1;
+}
+
+/* ------------------------------------------------------------------------------------
+ * Eigen/src/Core/ConditionEstimator.h:55-141 (rcond_invmatrix_L1_norm_estimate, real case)
+ * and :158-169 (rcond_estimate_helper) for an LLT decomposition: the solves use
+ * oracle_llt_solve with the lower factor l.  `work`: 4 n doubles. */
+static double norm1(const double *v, idx_t n)
+{
+    double s = 0.0;
+    for (idx_t i = 0; i < n; ++i) s += fabs(v[i]);
+    return s;
+}
+
+double oracle_llt_rcond(const double *l, idx_t n, idx_t ld, double anorm, double *work)
+{
+    if (n == 0) return INFINITY;
+    if (anorm == 0.0) return 0.0;
+    if (n == 1) return 1.0;
+    double *v = work, *sgn = work + n, *old_sgn = work + 2 * n;
+    for (idx_t i = 0; i < n; ++i) v[i] = 1.0 / (double)n;
+    oracle_llt_solve(l, n, ld, v, 1, n);
+    double lower = norm1(v, n), old_lower = lower;
+    idx_t old_j = -1;
+    for (int k = 0; k < 4; ++k) {
+        for (idx_t i = 0; i < n; ++i) sgn[i] = v[i] < 0.0 ? -1.0 : 1.0;
+        if (k > 0 && memcmp(sgn, old_sgn, (size_t)n * sizeof(double)) == 0) break;
+        memcpy(v, sgn, (size_t)n * sizeof(double));
+        oracle_llt_solve(l, n, ld, v, 1, n);
+        idx_t j = 0;
+        for (idx_t i = 1; i < n; ++i)
+            if (fabs(v[i]) > fabs(v[j])) j = i;
+        if (j == old_j) break;
+        for (idx_t i = 0; i < n; ++i) v[i] = 0.0;
+        v[j] = 1.0;
+        oracle_llt_solve(l, n, ld, v, 1, n);
+        lower = norm1(v, n);
+        if (lower <= old_lower) break;
+        memcpy(old_sgn, sgn, (size_t)n * sizeof(double));
+        old_j = j;
+        old_lower = lower;
+    }
+    double alt = 1.0;
+    for (idx_t i = 0; i < n; ++i) {
+        v[i] = alt * (1.0 + (double)i / (double)(n - 1));
+        alt = -alt;
+    }
+    oracle_llt_solve(l, n, ld, v, 1, n);
+    double alternate = (2.0 * norm1(v, n)) / (3.0 * (double)n);
+    double inv = lower > alternate ? lower : alternate;
+    return inv == 0.0 ? 0.0 : (1.0 / inv) / anorm;
+}
+
+/* ------------------------------------------------------------------------------------
+ * The level-3 routines of the EIGEN_USE_BLAS route, with the semantics of Eigen's BLAS
+ * (blas/level3_impl.h:78-178 trsm, :357-420 syrk): characters as blas/common.h:47-63 reads
+ * them ('C' is 'T' for real data).
+ *
+ * trsm: B <- alpha op(A)^{-1} B (side 'L', A m x m) or alpha B op(A)^{-1} (side 'R', A n x n),
+ * by plain substitution on the elements of the `uplo` triangle (diag 'U': ones). */
+static double tri_at(const double *a, idx_t lda, int upper, int unit, int trans, idx_t i, idx_t j)
+{
+    /* element (i, j) of op(A) restricted to its triangle; zero outside */
+    idx_t r = trans ? j : i, c = trans ? i : j;
+    if (upper ? (r > c) : (r < c)) return 0.0;
+    if (r == c && unit) return 1.0;
+    return a[r + c * lda];
+}
+
+void oracle_trsm(char side, char uplo, char transa, char diag, idx_t m, idx_t n, double alpha, const double *a,
+                 idx_t lda, double *b, idx_t ldb)
+{
+    int left = side == 'L' || side == 'l';
+    int upper = uplo == 'U' || uplo == 'u';
+    int trans = !(transa == 'N' || transa == 'n');
+    int unit = diag == 'U' || diag == 'u';
+    /* op(A) is lower triangular when (lower, no transpose) or (upper, transpose) */
+    int oplower = upper ? trans : !trans;
+    if (left) {
+        for (idx_t c = 0; c < n; ++c) {
+            double *x = b + c * ldb;
+            if (oplower) {
+                for (idx_t i = 0; i < m; ++i) {
+                    double s = x[i];
+                    for (idx_t k = 0; k < i; ++k) s -= tri_at(a, lda, upper, unit, trans, i, k) * x[k];
+                    x[i] = s / tri_at(a, lda, upper, unit, trans, i, i);
+                }
+            } else {
+                for (idx_t i = m - 1; i >= 0; --i) {
+                    double s = x[i];
+                    for (idx_t k = i + 1; k < m; ++k) s -= tri_at(a, lda, upper, unit, trans, i, k) * x[k];
+                    x[i] = s / tri_at(a, lda, upper, unit, trans, i, i);
+                }
+            }
+        }
+    } else { /* X op(A) = B, row by row: op(A)^T x^T = b^T */
+        for (idx_t r = 0; r < m; ++r) {
+            if (oplower) { /* op(A)^T upper: backward over columns */
+                for (idx_t j = n - 1; j >= 0; --j) {
+                    double s = b[r + j * ldb];
+                    for (idx_t k = j + 1; k < n; ++k) s -= b[r + k * ldb] * tri_at(a, lda, upper, unit, trans, k, j);
+                    b[r + j * ldb] = s / tri_at(a, lda, upper, unit, trans, j, j);
+                }
+            } else {
+                for (idx_t j = 0; j < n; ++j) {
+                    double s = b[r + j * ldb];
+                    for (idx_t k = 0; k < j; ++k) s -= b[r + k * ldb] * tri_at(a, lda, upper, unit, trans, k, j);
+                    b[r + j * ldb] = s / tri_at(a, lda, upper, unit, trans, j, j);
+                }
+            }
+        }
+    }
+    for (idx_t c = 0; c < n; ++c)
+        for (idx_t i = 0; i < m; ++i) b[i + c * ldb] *= alpha;
+}
+
+/* syrk: C <- alpha op(A) op(A)^T + beta C on the `uplo` triangle of C (n x n);
+ * op(A) = A (n x k) for 'N', A^T (A k x n) otherwise; beta == 0 does not read C. */
+void oracle_syrk(char uplo, char trans, idx_t n, idx_t k, double alpha, const double *a, idx_t lda, double beta,
+                 double *c, idx_t ldc)
+{
+    int upper = uplo == 'U' || uplo == 'u';
+    int notr = trans == 'N' || trans == 'n';
+    for (idx_t j = 0; j < n; ++j) {
+        idx_t lo = upper ? 0 : j, hi = upper ? j + 1 : n;
+        for (idx_t i = lo; i < hi; ++i) {
+            double s = 0.0;
+            for (idx_t q = 0; q < k; ++q) {
+                double ai = notr ? a[i + q * lda] : a[q + i * lda];
+                double aj = notr ? a[j + q * lda] : a[q + j * lda];
+                s += ai * aj;
+            }
+            double *p = c + i + j * ldc;
+            *p = alpha * s + (beta == 0.0 ? 0.0 : beta * *p);
+        }
+    }
+}

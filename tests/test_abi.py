@@ -28,7 +28,9 @@ def declared_functions():
 def test_headers_declare_the_boundary():
     names = declared_functions()
     for must in ("mi_llt_create", "mi_dpotrf", "mi_dpotrs", "mi_dposv_async", "mi_dposv_mixed", "LAPACKE_dpotrf",
-                 "LAPACKE_dpotrs", "dpotrf_", "dpotrs_"):
+                 "LAPACKE_dpotrs", "dpotrf_", "dpotrs_", "LAPACKE_spotrf", "LAPACKE_spotrs", "spotrf_", "spotrs_",
+                 "mi_spotrf", "mi_spotrs", "dtrsm_", "dsyrk_", "mi_dtrsm", "mi_dsyrk", "mi_dpocon",
+                 "mi_dllt_rank_update", "mi_llt_inject_status"):
         assert must in names
 
 
@@ -103,3 +105,26 @@ def test_product_path_has_no_oracle_import():
             if f.endswith((".py", ".hip", ".cpp", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "llt_oracle" not in text, f
+
+
+def test_blas_argument_checks():
+    """blas/level3_impl.h:139-149 (trsm) and :386-394 (syrk): the illegal argument is reported and
+    nothing is computed (no device work: these return before staging)."""
+    lib = mi_llt.lib()
+    I = ctypes.c_int  # noqa: E741
+    one = ctypes.c_double(1.0)
+    assert lib.dtrsm_(b"Q", b"L", b"N", b"N", ctypes.byref(I(3)), ctypes.byref(I(2)), ctypes.byref(one), None,
+                      ctypes.byref(I(3)), None, ctypes.byref(I(3))) == 0
+    assert lib.dsyrk_(b"L", b"Q", ctypes.byref(I(3)), ctypes.byref(I(2)), ctypes.byref(one), None, ctypes.byref(I(3)),
+                      ctypes.byref(one), None, ctypes.byref(I(3))) == 0
+    # the device entry points validate on the host too (-i counting the handle)
+    assert lib.mi_dtrsm(None, b"L", b"L", b"N", b"N", 3, 2, 1.0, None, 3, None, 3) == -1003
+
+
+def test_device_entry_point_argument_checks_need_no_gpu():
+    lib = mi_llt.lib()
+    assert lib.mi_spotrf(None, b"L", 4, None, 4, None) == -1003
+    assert lib.mi_dsyrk(None, b"L", b"N", 4, 2, 1.0, None, 4, 0.0, None, 4) == -1003
+    assert lib.mi_dllt_rank_update(None, b"L", 4, None, 4, None, 1, 1.0, None) == -1003
+    rc = ctypes.c_double(0.0)
+    assert lib.mi_dpocon(None, b"L", 4, None, 4, 1.0, ctypes.byref(rc)) == -1003
