@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/mi_llt.h"
+#include "factor_plan.hpp"
 #include "kernels.hpp"
 
 struct mi_llt_context {
@@ -180,59 +181,15 @@ int reserve_sync(mi_llt_handle h, size_t flag_words)
     return 0;
 }
 
-// Algorithmic flop counts (LAPACK convention): POTRF of b: b^3/3 + b^2/2 + b/6;
-// TRSM m x b panel: m b^2; SYRK lower m x m, depth b: m (m + 1) b; panel update m x b x b.
-//
-// Right-looking blocked Cholesky (llt_inplace::blocked, LLT.h:338-368) with a one-panel
-// lookahead, one launch per step on the caller's stream:
-//   launch -1 : panel 0 (diagonal factor + panel solve of block column 0)
-//   launch k  : panel k+1 (its update by panel k, diagonal factor, panel solve)
-//               + trailing update by panels k-1 and k (depth 256) of every other block
-//               column from k+2 on (factor_step.hip explains the alternation).
-// The two halves of a launch touch disjoint data; everything they read was finished by an
-// earlier launch, so stream order is the only synchronisation the host needs.
-double panel_flops(int64_t n, int64_t k0, int64_t NB)
-{
-    const double b = (double)((n - k0) < NB ? (n - k0) : NB);
-    const double m = (double)(n - k0 - NB > 0 ? n - k0 - NB : 0);
-    const double mc = (double)(n - k0);
-    return (k0 > 0 ? 2.0 * mc * b * NB - b * (b - 1) * NB : 0.0) + b * b * b / 3 + b * b / 2 + b / 6 + m * b * b;
-}
-
 template <typename T>
 int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, int *d_info, int prec = MI_PREC_FP64,
                  __bf16 *wsplit = nullptr)
 {
-    constexpr int64_t NB = mi::kNB;
-    const int64_t nblk = (n + NB - 1) / NB;
     int rc = reserve_sync(h, 0);
     if (rc) return rc;
     hipStream_t s = h->stream;
-    // ticket counter and panel flag start from zero for every factorization (a kernel, not a
-    // memset: the factorization may be captured in a graph); the ticket bases and the flag
-    // epochs below count from there, so they are the same in every call and every replay
-    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, h->sync + 4, 2);
-    uint32_t tickets = 0, epoch_now = 0;
-    for (int64_t k = -1; k < nblk; ++k) {
-        const int64_t kp = (k + 1 < nblk) ? (k + 1) * NB : -1;  // panel of this launch
-        // trailing update of launch k: panels k-1, k (k >= 1; panel 0 alone for k = 0) to the
-        // block columns c = k+2, k+4, ... (all rows from c down)
-        const double K = k >= 1 ? 2.0 * NB : (double)NB;
-        double fl_upd = 0.0;
-        for (int64_t c = k + 2; k >= 0 && c < nblk; c += 2) {
-            const double m = (double)(n - c * NB), w = m < NB ? m : (double)NB;
-            fl_upd += 2.0 * K * (m * w - w * (w - 1) / 2);
-        }
-        if (kp < 0 && fl_upd == 0.0) break;
-        const double fl = (kp >= 0 ? panel_flops(n, kp, NB) : 0.0) + fl_upd;
-        const uint32_t epoch = (kp >= 0) ? ++epoch_now : epoch_now;
-        uint32_t grid = 0;
-        traced(h, s, MI_TRACE_POTRF, fl, [&] {
-            mi::launch_factor_step<T>(A, lda, n, kp, k, wleaf, d_info, h->sync + 4, tickets, h->sync + 5, epoch,
-                                      (int *)h->sync, s, &grid, prec, wsplit);
-        });
-        tickets += grid;
-    }
+    mi::enqueue_factorization<T>(s, A, lda, n, wleaf, d_info, h->sync, prec, wsplit,
+                                 [&](double fl, auto &&launch) { traced(h, s, MI_TRACE_POTRF, fl, launch); });
     return 0;
 }
 

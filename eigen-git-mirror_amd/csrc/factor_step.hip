@@ -20,6 +20,7 @@
 // (the critical path, always running before anyone waits for it), the trailing-update tiles,
 // then the panel's row roles -- which wait for the diagonal block, so they should not hold
 // CUs while it is being factored.
+#include "factor_plan.hpp"
 #include "potrf_device.hpp"
 #include "syrk_device.hpp"
 
@@ -97,6 +98,16 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
     }
     syrk_tile<K, S, PREC>(A + r + r * lda, lda, A + r + slab * lda, lda, (int)m, (int)(m < kNB ? m : kNB), i - c,
                           0);
+}
+
+__global__ void factor_reset_kernel(uint32_t *sync)
+{
+    if (threadIdx.x < 2) sync[4 + threadIdx.x] = 0u;
+}
+
+void launch_factor_reset(uint32_t *sync, hipStream_t s)
+{
+    hipLaunchKernelGGL(factor_reset_kernel, dim3(1), dim3(64), 0, s, sync);
 }
 
 // Trailing-update tiles of launch k (panel k just finished; k = -1: none).

@@ -33,8 +33,9 @@ def test_cholesky_adjoint_matches_torch(dev, n):
     assert (g1 - g2).abs().max() <= 1e-10 * g2.abs().max()
 
 
-def test_cholesky_adjoint_finite_differences(dev):
-    n = 24
+@pytest.mark.parametrize("n,samples", [(24, 60), (130, 24), (300, 12)])
+def test_cholesky_adjoint_finite_differences(dev, n, samples):
+    print(f"[gpu] adjoint finite differences n={n}", flush=True)
     a = spd(n, 3, dev)
     w = torch.from_numpy(np.tril(np.random.default_rng(1).standard_normal((n, n)))).to(dev)
     a1 = a.clone().requires_grad_(True)
@@ -43,7 +44,7 @@ def test_cholesky_adjoint_finite_differences(dev):
     f = lambda m: float((mad.cholesky(m) * w).sum())  # noqa: E731
     eps = 1e-6
     rng = np.random.default_rng(2)
-    for _ in range(12):
+    for _ in range(samples):
         i, j = sorted(rng.integers(0, n, size=2))[::-1]
         e = torch.zeros_like(a)
         e[i, j] = 1.0
@@ -73,13 +74,10 @@ def test_gp_gradient_closed_form(dev, n):
     k1 = k.clone().requires_grad_(True)
     ll = mad.gp_log_marginal(k1, y)
     ll.backward()
-    # closed form with the library's own solve
-    l = k.clone()
-    assert mi_llt.potrf_(l) == 0
-    alpha = mi_llt.colmajor(y.reshape(-1, 1))
-    mi_llt.potrs_(l, alpha)
-    kinv = mi_llt.colmajor(torch.eye(n, dtype=torch.float64, device=dev))
-    mi_llt.potrs_(l, kinv)
+    # closed form from torch's FP64 inverse (an independent library routine, checker only)
+    ksym = torch.tril(k) + torch.tril(k, -1).t()
+    kinv = torch.linalg.inv(ksym)
+    alpha = (kinv @ y).reshape(-1, 1)
     ref = 0.5 * (alpha @ alpha.t() - kinv)
     err = float((k1.grad - ref).abs().max() / ref.abs().max())
     assert err <= 1e-9, err

@@ -254,7 +254,9 @@ def test_rank_update_matches_oracle(dev, n, sigma):
     say(f"[gpu] rank update n={n} sigma={sigma}")
     a = workloads.gram_spd(n, seed=n + 17)
     lo = np.tril(oracle.llt_blocked(a)[0])
-    v = workloads.rhs(n, 1, seed=n).reshape(-1) * (0.1 if sigma < 0 else 1.0)
+    # a downdate keeps A + sigma v v^T positive definite when |sigma| ||v||^2 is below the
+    # smallest eigenvalue (~0.1 here): ||v||^2 ~ 0.09 for the downdates
+    v = workloads.rhs(n, 1, seed=n).reshape(-1) * (0.3 / np.sqrt(n) if sigma < 0 else 1.0)
     ref, idx = oracle.llt_rank_update(lo, v, sigma)
     assert idx == -1
     junk = lo + 3.0 * np.triu(np.ones((n, n)), 1)  # the other triangle must stay as it is
@@ -270,14 +272,16 @@ def test_rank_update_matches_oracle(dev, n, sigma):
     assert np.linalg.norm(lu @ lu.T - target) / np.linalg.norm(target) <= 1e-14
 
 
-@pytest.mark.parametrize("n,scale", [(60, 30.0), (300, 3.0), (1000, 2.0)])
-def test_rank_downdate_failing_column(dev, n, scale):
+@pytest.mark.parametrize("n,k0,scale", [(60, 40, 30.0), (300, 200, 3.0), (1000, 700, 2.0)])
+def test_rank_downdate_failing_column(dev, n, k0, scale):
     say(f"[gpu] rank downdate failure n={n}")
     a = workloads.gram_spd(n, seed=n)
     lo = np.tril(oracle.llt_blocked(a)[0])
+    # v is zero above row k0: the first k0 columns go through unchanged, the failure comes later
     v = workloads.rhs(n, 1, seed=n + 1).reshape(-1) * scale
+    v[:k0] = 0.0
     ref, idx = oracle.llt_rank_update(lo, v, -1.0)
-    assert idx >= 0
+    assert idx >= k0
     dl = dev_t(lo, dev)
     assert mi_llt.rank_update_(dl, torch.from_numpy(v).to(dev), -1.0) == idx + 1
     got = dl.cpu().numpy()
