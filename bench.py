@@ -50,29 +50,61 @@ def log(*a):
 
 
 def cpu_baseline(a_host, b_host, n, r):
-    """Genuine Eigen LLT<MatrixXd>::compute + solve (oracle/_ref) on this host, one run of
-    the full workload; falls back to the oracle's C restatement when the Eigen build is
-    absent.  Eigen's LLT never reaches its OpenMP GEMM (SelfadjointProduct.h:106,
-    SolveTriangular.h:98), so it uses one core."""
+    """Genuine Eigen LLT<MatrixXd>::compute + solve (oracle/_ref) on this host, one run of the
+    full workload with Eigen allowed 1 thread and one with every host thread (Eigen's LLT never
+    reaches its parallel GEMM -- SelfadjointProduct.h:106, SolveTriangular.h:98 -- so both use
+    one core; the faster run is reported, with its thread setting); falls back to the oracle's
+    C restatement when the Eigen build is absent."""
     import oracle
 
+    fl = flops_factor(n) + flops_solve(n, r)
     if oracle.have_eigen():
-        t = oracle.eigen_llt_time(a_host, b_host)
-        kind = "reference"
+        runs = {}
+        for threads in (1, os.cpu_count() or 1):
+            got = oracle.eigen_set_threads(threads)
+            runs[got] = oracle.eigen_llt_time(a_host, b_host)
+        best = min(runs, key=runs.get)
+        t = runs[best]
         what = f"Eigen {oracle._load_eigen().eigen_ref_version()} LLT<MatrixXd> compute()+solve(), " \
                f"{os.path.basename(oracle.eigen_path())}"
-    else:
-        t0 = time.perf_counter()
-        m, _ = oracle.llt_blocked(a_host)
-        oracle.llt_solve(m, b_host)
-        t = time.perf_counter() - t0
-        kind = "port"
-        what = "oracle C restatement of llt_inplace::blocked + solve"
+        return {"value": fl / t / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "reference",
+                "sample": f"one full n={n}, {r}-RHS factor+solve ({what}); wall "
+                          + ", ".join(f"{v:.3f} s with Eigen::setNbThreads({k})" for k, v in sorted(runs.items()))
+                          + f" -- the LLT path runs on one core either way; host threads {os.cpu_count()}",
+                "seconds": t, "seconds_by_threads": {str(k): v for k, v in runs.items()}}
+    t0 = time.perf_counter()
+    m, _ = oracle.llt_blocked(a_host)
+    oracle.llt_solve(m, b_host)
+    t = time.perf_counter() - t0
+    return {"value": fl / t / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"one full n={n}, {r}-RHS factor+solve (oracle C restatement of llt_inplace::blocked + "
+                      f"solve, one thread); wall {t:.3f} s", "seconds": t}
+
+
+def host_api_rate(a_host, b_host, n, r):
+    """The drop-in path end to end: LAPACKE_dpotrf then LAPACKE_dpotrs on host (pageable)
+    arrays, each staging its operands over PCIe (what an unmodified Eigen built with
+    EIGEN_USE_LAPACKE pays per LLT::compute); one warm call, then one timed.  Reported beside
+    the device-resident value, never as it."""
+    import numpy as np
+
+    import mi_llt
+
+    lib = mi_llt.lib()
     fl = flops_factor(n) + flops_solve(n, r)
-    return {"value": fl / t / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"one full n={n}, {r}-RHS factor+solve ({what}); wall {t:.3f} s; "
-                      f"host threads available {os.cpu_count()}, used by LLT 1",
-            "seconds": t}
+    times = []
+    for _ in range(2):
+        a = np.asfortranarray(a_host.copy())
+        b = np.asfortranarray(b_host.copy())
+        t0 = time.perf_counter()
+        i1 = lib.LAPACKE_dpotrf(102, b"L", n, a.ctypes.data, n)
+        i2 = lib.LAPACKE_dpotrs(102, b"L", n, r, a.ctypes.data, n, b.ctypes.data, n)
+        times.append(time.perf_counter() - t0)
+        if i1 != 0 or i2 != 0:
+            raise SystemExit(f"LAPACKE shims returned {i1}, {i2}")
+    t = times[-1]
+    return {"value": round(fl / t / 1e12, 4), "unit": "TFLOP/s", "ms": round(t * 1e3, 2),
+            "what": "LAPACKE_dpotrf + LAPACKE_dpotrs on host arrays (H2D/D2H of A twice, B once each way)"}
 
 
 def _time_ms(fn, reps, setup=None):
@@ -236,16 +268,87 @@ def leg_adjoint(n=4096, nb=512, batch=256, reps=3):
                      "adjoint_tflops_performed": round(adj_done / t_adj / 1e9, 3),
                      "adjoint_frac_fp64_peak_performed": round(adj_done / t_adj / 1e9 / PEAK_FP64_TFLOPS, 4)}
     ks = [workloads.gp_kernel_torch(nb, seed=100 + i, device=dev) for i in range(batch)]
+    streams = [torch.cuda.Stream() for _ in range(4)]
 
     def batch_all():
-        for kk, yy in ks:
-            k1 = kk.detach().requires_grad_(True)
-            mad.gp_log_marginal(k1, yy).backward()
+        # independent problems on 4 streams (one library handle per stream)
+        main = torch.cuda.current_stream()
+        for st in streams:
+            st.wait_stream(main)
+        for i, (kk, yy) in enumerate(ks):
+            with torch.cuda.stream(streams[i % 4]):
+                k1 = kk.detach().requires_grad_(True)
+                mad.gp_log_marginal(k1, yy).backward()
+        for st in streams:
+            main.wait_stream(st)
 
     batch_all()
-    tb, _ = _time_ms(batch_all, 1)
-    out["batch"] = {"count": batch, "n": nb, "forward_backward_ms": round(tb, 3)}
+    tb, _ = _time_ms(batch_all, 2)
+    # forward F(n) + adjoint n^3/3 + 2 n^3 per system (LAPACK / textbook counts)
+    fl_b = batch * (flops_factor(nb) + nb ** 3 / 3 + 2.0 * nb ** 3)
+    out["batch"] = {"count": batch, "n": nb, "streams": 4, "forward_backward_ms": round(tb, 3),
+                    "tflops": round(fl_b / tb / 1e9, 3)}
     return out
+
+
+def counter_workload(name: str, reps: int):
+    """A short, fixed workload for hardware-counter runs (rocprofv3 --pmc ... -- python3 bench.py
+    --counter-workload NAME): first two 256 MiB device copies (the calibration kernel of the
+    FETCH_SIZE / WRITE_SIZE scale factors), then `reps` calls of one path.  Prints nothing on
+    stdout (no bench line)."""
+    import torch
+
+    import mi_llt
+    from mi_llt import autograd as mad
+    from mi_llt import batched, workloads
+
+    dev = torch.device("cuda:0")
+    src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    h = mi_llt.Handle().bind_stream()
+    if name in ("factor_fp64", "solve_fp64"):  # the headline workload: n = 8192, 64 RHS
+        n, r = 8192, 64
+        a = workloads.gram_spd_torch(n, seed=0, device=dev)
+        b = mi_llt.colmajor(torch.from_numpy(workloads.rhs(n, r, seed=1)).to(dev))
+        da, db = a.clone(), b.clone()
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        for _ in range(reps):
+            da.copy_(a)
+            db.copy_(b)
+            mi_llt.posv_async_(da, db, info, handle=h)
+    elif name in ("mixed_bf16x3", "mixed_bf16x6", "mixed_fp32"):  # n = 16384, kappa 1e2, 16 RHS
+        n, r = 16384, 16
+        a = workloads.spectrum_spd_torch(n, 1e2, seed=0, device=dev)
+        b = mi_llt.colmajor(torch.from_numpy(workloads.rhs(n, r, seed=1)).to(dev))
+        for _ in range(reps):
+            mi_llt.posv_mixed(a, b, prec=name.split("_")[1], tol=1e-14, maxit=30, handle=h)
+    elif name in ("vbatched_fp64", "vbatched_fp32"):  # 20,000 uneven systems
+        dt = torch.float64 if name.endswith("64") else torch.float32
+        vb = batched.gram_batch(20000, seed=0, dtype=dt, device=dev)
+        a0, b0 = vb.a_flat.clone(), vb.b_flat.clone()
+        for _ in range(reps):
+            vb.a_flat.copy_(a0)
+            vb.b_flat.copy_(b0)
+            batched.posv_vbatched_(vb, h)
+    elif name == "adjoint":  # GP gradient, n = 4096
+        k, y = workloads.gp_kernel_torch(4096, seed=0, device=dev)
+        for _ in range(reps):
+            k1 = k.detach().requires_grad_(True)
+            mad.gp_log_marginal(k1, y).backward()
+    else:
+        raise SystemExit(f"unknown counter workload {name!r}")
+    torch.cuda.synchronize()
+    if h.status() != 0:
+        raise SystemExit("device status reported a failure")
+    log(f"[bench] counter workload {name}: {reps} calls done")
+
+
+COUNTER_WORKLOADS = ("factor_fp64", "solve_fp64", "mixed_bf16x3", "mixed_bf16x6", "mixed_fp32", "vbatched_fp64",
+                     "vbatched_fp32", "adjoint")
 
 
 def main():
@@ -257,7 +360,14 @@ def main():
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-legs", action="store_true", help="skip the mixed / batched / adjoint legs")
+    ap.add_argument("--no-host-api", action="store_true", help="skip the PCIe-inclusive LAPACKE drop-in timing")
+    ap.add_argument("--counter-workload", choices=COUNTER_WORKLOADS,
+                    help="run a short fixed workload for rocprofv3 counter runs instead of the bench")
+    ap.add_argument("--reps", type=int, default=2, help="calls of the counter workload")
     args = ap.parse_args()
+    if args.counter_workload:
+        counter_workload(args.counter_workload, args.reps)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X and the path is built for one GPU "
               f"(got --gpus {args.gpus}).", file=sys.stderr)
@@ -347,9 +457,16 @@ def main():
                                "launches": st_n}}
 
     cpu = None
-    if not args.no_cpu:
-        log("[bench] cpu baseline (Eigen LLT on the host)")
-        cpu = cpu_baseline(np.asfortranarray(a.cpu().numpy()), np.asfortranarray(b.cpu().numpy()), n, r)
+    host_api = None
+    if not args.no_cpu or not args.no_host_api:
+        a_host, b_host = np.asfortranarray(a.cpu().numpy()), np.asfortranarray(b.cpu().numpy())
+        if not args.no_host_api:
+            log("[bench] host-API (LAPACKE drop-in, PCIe-inclusive) timing")
+            host_api = host_api_rate(a_host, b_host, n, r)
+        if not args.no_cpu:
+            log("[bench] cpu baseline (Eigen LLT on the host)")
+            cpu = cpu_baseline(a_host, b_host, n, r)
+        del a_host, b_host
 
     legs = None
     if not args.no_legs:
@@ -369,6 +486,7 @@ def main():
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "host_api": host_api,
         "wall_s_timed": round(t_wall, 4),
         "legs": legs,
     }

@@ -117,6 +117,8 @@ def _load_eigen():
                                                    _DP, _I64, _DP, _DP, _DP, _DP]
         lib.eigen_ref_llt_time.restype = ctypes.c_double
         lib.eigen_ref_llt_time.argtypes = [_I64, _DP, _I64, _DP, _I64, _I64, _DP, _I64]
+        lib.eigen_ref_set_threads.restype = ctypes.c_int
+        lib.eigen_ref_set_threads.argtypes = [ctypes.c_int]
         lib.eigen_ref_llt_inplace_f32.restype = _I64
         lib.eigen_ref_llt_inplace_f32.argtypes = [_I64, ctypes.POINTER(ctypes.c_float), _I64]
         lib.eigen_ref_llt_rank_update.restype = _I64
@@ -245,6 +247,11 @@ def eigen_llt(a: np.ndarray, b: np.ndarray):
         ctypes.byref(tf), ctypes.byref(ts), None, ctypes.byref(rc))
     return {"L": l, "X": x.reshape(-1) if vec else x, "info": int(info),
             "t_factor": tf.value, "t_solve": ts.value, "rcond": rc.value}
+
+
+def eigen_set_threads(n: int) -> int:
+    """Eigen::setNbThreads(n); returns Eigen::nbThreads()."""
+    return int(_load_eigen().eigen_ref_set_threads(int(n)))
 
 
 def eigen_llt_time(a: np.ndarray, b: np.ndarray) -> float:

@@ -27,6 +27,14 @@ static double now_s()
 
 extern "C" {
 
+// Threads Eigen's parallel products may use (Eigen::setNbThreads, Parallelizer.h:50-59);
+// returns the number Eigen then reports.
+int eigen_ref_set_threads(int n)
+{
+    Eigen::setNbThreads(n);
+    return Eigen::nbThreads();
+}
+
 // Version string of the compiled Eigen (Macros.h:18-20).
 int eigen_ref_version(void)
 {
