@@ -60,11 +60,19 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
                                                           int *status, __bf16 *Wsplit, int64_t slot_elems)
 {
     __shared__ __align__(16) double smem[kSmem];
-    __shared__ int s_role, s_ok;
-    // every workgroup draws its ticket, even when it then stops: the host counts them
-    if (threadIdx.x == 0) s_role = (int)(atomicAdd(tickets, 1u) - ticket_base);
+    __shared__ int s_role, s_ok, s_stop;
+    // Every workgroup draws its ticket, even when it then stops: the host counts them.  The
+    // failure words (info, status) are read in the same round trip as the ticket, so a
+    // workgroup starts after one memory latency, not three.
+    if (threadIdx.x == 0) {
+        const uint32_t tk = atomicAdd(tickets, 1u);
+        const int iv = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int sv = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_role = (int)(tk - ticket_base);
+        s_stop = iv | sv;
+    }
     __syncthreads();
-    if (factor_failed(info) || device_failed(status)) return;
+    if (s_stop) return;
     const int t = __builtin_amdgcn_readfirstlane(s_role);  // uniform: every derived address is scalar
     // A role outside this launch means the counter was not reset with the host's accounting
     // (never in a correct call sequence): record it and touch nothing, rather than turn the
