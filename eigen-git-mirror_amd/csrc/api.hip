@@ -29,8 +29,8 @@ struct mi_llt_context {
     void *ws = nullptr;       // grow-only device workspace
     size_t ws_bytes = 0;
     int *d_info = nullptr;    // device int for the synchronous entry points
-    // synchronisation words: [status | 2 solve tickets | - | factor tickets | panel flag | ...];
-    // zeroed on the handle's stream when allocated; the ticket and flag words of a solve or a
+    // synchronisation words (status, solve and rank-update tickets, factor tickets, panel flag
+    // at the mi::kSync* offsets, each group on its own cache line); zeroed on the handle's stream when allocated; the ticket and flag words of a solve or a
     // factorization are reset on the stream at its start, so every call (and every replay of a
     // captured call) counts from zero
     uint32_t *sync = nullptr;
@@ -156,7 +156,7 @@ int prepare(mi_llt_handle h, int64_t n, bool upper, Layout &lay)
     return 0;
 }
 
-constexpr size_t kSyncHeader = 8;
+constexpr size_t kSyncHeader = mi::kSyncWords;
 
 int reserve_sync(mi_llt_handle h, size_t flag_words)
 {
@@ -226,7 +226,8 @@ int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, i
     }
     const double per_sweep = (double)n * (double)n * (double)nrhs;
     traced(h, h->stream, MI_TRACE_SOLVE, per_sweep * ((sweeps & 1) + ((sweeps >> 1) & 1)), [&] {
-        mi::launch_solve_persistent<T>(L, ldl, n, B, ldb, nrhs, winv, h->ypk, (int *)h->sync + 1, d_info, (int *)h->sync,
+        mi::launch_solve_persistent<T>(L, ldl, n, B, ldb, nrhs, winv, h->ypk, (int *)h->sync + mi::kSyncSolveTickets, d_info,
+                                       (int *)h->sync + mi::kSyncStatus,
                                        h->stream, sweeps, upper_copy);
     });
     return 0;
@@ -352,11 +353,11 @@ int read_status(mi_llt_handle h, int *status)
     if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
     if (!h->sync) return 0;
     int v = 0;
-    e = hipMemcpyAsync(&v, h->sync, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    e = hipMemcpyAsync(&v, h->sync + mi::kSyncStatus, sizeof(int), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(h, e, "status readback");
     if (v != 0) {  // cleared on the stream: ordered before any later work of the handle
-        hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, (int *)h->sync, 0);
+        hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, h->stream, (int *)h->sync + mi::kSyncStatus, 0);
         e = hipGetLastError();
         if (e != hipSuccess) return fail_hip(h, e, "status clear");
     }
@@ -468,7 +469,7 @@ int mi_llt_inject_status(mi_llt_handle h, int bits)
     if (!valid(h)) return MI_LLT_E_HANDLE;
     int rc = reserve_sync(h, 0);
     if (rc) return rc;
-    hipLaunchKernelGGL(or_int_kernel, dim3(1), dim3(1), 0, h->stream, (int *)h->sync, bits);
+    hipLaunchKernelGGL(or_int_kernel, dim3(1), dim3(1), 0, h->stream, (int *)h->sync + mi::kSyncStatus, bits);
     return check_launch(h, "inject status");
 }
 
@@ -817,7 +818,8 @@ int mi_dllt_rank_update(mi_llt_handle h, char uplo, int64_t n, double *dL, int64
         ld = n;
         mi::launch_upper_to_lower(dL, ldl, n, L, ld, s);
     }
-    mi::launch_rank_update(L, ld, n, dv, incv, sigma, h->ws, (int *)h->sync + 3, d_info, (int *)h->sync, s);
+    mi::launch_rank_update(L, ld, n, dv, incv, sigma, h->ws, (int *)h->sync + mi::kSyncRankTicket, d_info,
+                           (int *)h->sync + mi::kSyncStatus, s);
     if (up == 1) mi::launch_lower_to_upper<double>(L, ld, n, dL, ldl, s);
     return check_launch(h, "rank_update");
 }

@@ -14,8 +14,9 @@
 
 namespace mi {
 
-// Zeroes the ticket counter and the panel flag (sync[4], sync[5]) on the stream: a kernel, not
-// a memset, so that a captured factorization replays it (factor_step.hip).
+// Zeroes the ticket counter and the panel flag (sync[kSyncFactorTickets], sync[kSyncFlag]) on
+// the stream: a kernel, not a memset, so that a captured factorization replays it
+// (factor_step.hip).
 void launch_factor_reset(uint32_t *sync, hipStream_t s);
 
 // Algorithmic flop counts (LAPACK convention): POTRF of b: b^3/3 + b^2/2 + b/6; TRSM m x b
@@ -29,7 +30,7 @@ inline double panel_flops(int64_t n, int64_t k0, int64_t NB)
 }
 
 // Queues the factorization of the lower triangle of A (n x n, ld lda) on stream s.
-// sync: the synchronisation words ([0] status, [4] tickets, [5] panel flag); wleaf: the panel
+// sync: the synchronisation words (status, tickets, panel flag at kSync*); wleaf: the panel
 // images (kPanelDoubles per 128-block).  each(flops, launch) runs every step's launch (the
 // tracing hook of the handle API); the ticket bases and flag epochs count from the reset, so
 // every call and every replay of a captured call has the same accounting.
@@ -56,8 +57,8 @@ void enqueue_factorization(hipStream_t s, T *A, int64_t lda, int64_t n, double *
         const uint32_t epoch = (kp >= 0) ? ++epoch_now : epoch_now;
         uint32_t grid = 0;
         each(fl, [&] {
-            launch_factor_step<T>(A, lda, n, kp, k, wleaf, d_info, sync + 4, tickets, sync + 5, epoch, (int *)sync, s,
-                                  &grid, prec, wsplit);
+            launch_factor_step<T>(A, lda, n, kp, k, wleaf, d_info, sync + kSyncFactorTickets, tickets, sync + kSyncFlag,
+                                  epoch, (int *)sync + kSyncStatus, s, &grid, prec, wsplit);
         });
         tickets += grid;
     }

@@ -110,7 +110,8 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
 
 __global__ void factor_reset_kernel(uint32_t *sync)
 {
-    if (threadIdx.x < 2) sync[4 + threadIdx.x] = 0u;
+    if (threadIdx.x == 0) sync[kSyncFactorTickets] = 0u;
+    if (threadIdx.x == 1) sync[kSyncFlag] = 0u;
 }
 
 void launch_factor_reset(uint32_t *sync, hipStream_t s)

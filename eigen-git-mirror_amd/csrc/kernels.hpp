@@ -25,6 +25,17 @@ constexpr int64_t kInvDoubles = 6 * 128 * 128;
 constexpr int64_t kInvX = 128 * 128, kInvM = 2 * 128 * 128, kInvN = 3 * 128 * 128;  // offsets
 constexpr int64_t kInvM2 = 4 * 128 * 128, kInvN2 = 5 * 128 * 128;
 
+// Synchronisation words of a handle (32-bit words in device memory).  Each group sits on its
+// own 128-byte line: every starting workgroup of a factorization step takes an atomic on the
+// ticket counter and the waiting ones poll the flag, and a read of another word of that line
+// (the status, read by every workgroup as it starts) waits behind those atomics.
+constexpr int kSyncStatus = 0;          // status bits (MI_SOLVE_TIMEOUT, ...)
+constexpr int kSyncSolveTickets = 32;   // 2 words: the forward and the backward sweep
+constexpr int kSyncRankTicket = 34;     // the rank update's ticket
+constexpr int kSyncFactorTickets = 64;  // the factorization's ticket counter
+constexpr int kSyncFlag = 96;           // the factorization's panel flag
+constexpr int kSyncWords = 128;
+
 // Device status bits (a word in the handle's persistent device memory).
 constexpr int MI_SOLVE_TIMEOUT = 1;
 constexpr int MI_PANEL_TIMEOUT = 2;

@@ -6,7 +6,7 @@
 // launch_solve_persistent); only where their scratch lives differs.  The array `scratch` is
 // resolved once per device with hipGetSymbolAddress (the host-side name is a stand-in, not a
 // device address), and its device address is what every kernel receives.  Layout of one call:
-//   [ sync words (256 B): status, solve tickets, factor tickets, panel flag
+//   [ sync words (512 B): status, solve tickets, factor tickets, panel flag, each on its own line
 //   | panel images (kPanelDoubles per 128-block)
 //   | block inverses + couplings (kInvDoubles per 128-block; only when nrhs > 0)
 //   | the solve's packed block copies (2 * solve_pack_doubles(n, nrhs); only when nrhs > 0) ]
@@ -28,7 +28,7 @@ static __device__ __align__(256) unsigned char scratch[MI_NOALLOC_SCRATCH_BYTES]
 namespace {
 
 constexpr size_t kScratchBytes = (size_t)MI_NOALLOC_SCRATCH_BYTES;
-constexpr size_t kSyncBytes = 256;
+constexpr size_t kSyncBytes = 4 * mi::kSyncWords;
 constexpr int kMaxDevices = 64;
 
 // Device address of `scratch` per device, resolved on the first call made with that device
@@ -40,8 +40,8 @@ size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 __global__ void noalloc_begin_kernel(uint32_t *sync, int *info)
 {
-    // status, the two solve tickets, the factor tickets and the panel flag start at zero
-    if (threadIdx.x < 8) sync[threadIdx.x] = 0u;
+    // status, the solve tickets, the factor tickets and the panel flag start at zero
+    if (threadIdx.x < mi::kSyncWords) sync[threadIdx.x] = 0u;
     if (threadIdx.x == 0) *info = 0;
 }
 
@@ -49,7 +49,7 @@ __global__ void noalloc_begin_kernel(uint32_t *sync, int *info)
 // becomes the call's info, as no status query exists for this entry point.
 __global__ void noalloc_end_kernel(const uint32_t *sync, int *info)
 {
-    if (threadIdx.x == 0 && __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+    if (threadIdx.x == 0 && __hip_atomic_load(sync + mi::kSyncStatus, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
         *info = MI_LLT_E_DEVICE;
 }
 
@@ -90,14 +90,14 @@ extern "C" int mi_dposv_noalloc(char uplo, int64_t n, int64_t nrhs, double *dA, 
     double *winv = (double *)(base + kSyncBytes + leaf);
     double *ypk = (double *)(base + kSyncBytes + leaf + inv);
 
-    hipLaunchKernelGGL(noalloc_begin_kernel, dim3(1), dim3(64), 0, s, sync, d_info);
+    hipLaunchKernelGGL(noalloc_begin_kernel, dim3(1), dim3(mi::kSyncWords), 0, s, sync, d_info);
     if (n > 0) {
         mi::enqueue_factorization<double>(s, dA, lda, n, wleaf, d_info, sync, MI_PREC_FP64, nullptr,
                                           [](double, auto &&launch) { launch(); });
         if (nrhs > 0) {
             mi::launch_block_inverses<double>(dA, lda, n, wleaf, winv, d_info, s);
-            mi::launch_solve_persistent<double>(dA, lda, n, dB, ldb, nrhs, winv, ypk, (int *)sync + 1, d_info,
-                                                (int *)sync, s);
+            mi::launch_solve_persistent<double>(dA, lda, n, dB, ldb, nrhs, winv, ypk, (int *)sync + mi::kSyncSolveTickets,
+                                                d_info, (int *)sync + mi::kSyncStatus, s);
         }
     }
     hipLaunchKernelGGL(noalloc_end_kernel, dim3(1), dim3(64), 0, s, sync, d_info);
