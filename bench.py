@@ -267,27 +267,34 @@ def leg_adjoint(n=4096, nb=512, batch=256, reps=3):
                      "adjoint_frac_fp64_peak": round(adj_fl / t_adj / 1e9 / PEAK_FP64_TFLOPS, 4),
                      "adjoint_tflops_performed": round(adj_done / t_adj / 1e9, 3),
                      "adjoint_frac_fp64_peak_performed": round(adj_done / t_adj / 1e9 / PEAK_FP64_TFLOPS, 4)}
-    ks = [workloads.gp_kernel_torch(nb, seed=100 + i, device=dev) for i in range(batch)]
-    streams = [torch.cuda.Stream() for _ in range(4)]
+    ks, ys = zip(*[workloads.gp_kernel_torch(nb, seed=100 + i, device=dev) for i in range(batch)])
+    kb = torch.stack(ks)
+    yb = torch.stack(ys)
 
     def batch_all():
-        # independent problems on 4 streams (one library handle per stream)
-        main = torch.cuda.current_stream()
-        for st in streams:
-            st.wait_stream(main)
-        for i, (kk, yy) in enumerate(ks):
-            with torch.cuda.stream(streams[i % 4]):
-                k1 = kk.detach().requires_grad_(True)
-                mad.gp_log_marginal(k1, yy).backward()
-        for st in streams:
-            main.wait_stream(st)
+        # one batched forward + backward: every kernel launch covers all systems
+        k1 = kb.detach().requires_grad_(True)
+        mad.gp_log_marginal_batched(k1, yb).sum().backward()
+        return k1.grad
+
+    def batch_fwd():
+        with torch.no_grad():
+            mad.gp_log_marginal_batched(kb, yb)
 
     batch_all()
-    tb, _ = _time_ms(batch_all, 2)
-    # forward F(n) + adjoint n^3/3 + 2 n^3 per system (LAPACK / textbook counts)
-    fl_b = batch * (flops_factor(nb) + nb ** 3 / 3 + 2.0 * nb ** 3)
-    out["batch"] = {"count": batch, "n": nb, "streams": 4, "forward_backward_ms": round(tb, 3),
-                    "tflops": round(fl_b / tb / 1e9, 3)}
+    tb, _ = _time_ms(batch_all, reps)
+    tbf, _ = _time_ms(batch_fwd, reps)
+    tba = max(tb - tbf, 1e-6)
+    # forward F(n) + 2 n^2 (solve) and adjoint n^3/3 + 2 n^3 per system (LAPACK / textbook counts)
+    fl_f = batch * (flops_factor(nb) + 2.0 * nb * nb)
+    fl_a = batch * (nb ** 3 / 3 + 2.0 * nb ** 3)
+    out["batch"] = {"count": batch, "n": nb, "forward_backward_ms": round(tb, 3), "forward_ms": round(tbf, 3),
+                    "adjoint_ms": round(tba, 3), "tflops": round((fl_f + fl_a) / tb / 1e9, 3),
+                    "frac_fp64_peak": round((fl_f + fl_a) / tb / 1e9 / PEAK_FP64_TFLOPS, 4),
+                    "adjoint_tflops": round(fl_a / tba / 1e9, 3),
+                    "adjoint_frac_fp64_peak": round(fl_a / tba / 1e9 / PEAK_FP64_TFLOPS, 4),
+                    "path": "mi_dpotrf_batched_async / mi_dpotrs_batched forward, "
+                            "mi_dpotrs_adjoint_batched / mi_dpotrf_adjoint_batched backward"}
     return out
 
 
@@ -339,6 +346,12 @@ def counter_workload(name: str, reps: int):
         for _ in range(reps):
             k1 = k.detach().requires_grad_(True)
             mad.gp_log_marginal(k1, y).backward()
+    elif name == "adjoint_batch":  # GP gradients, 256 systems of n = 512
+        ks, ys = zip(*[workloads.gp_kernel_torch(512, seed=100 + i, device=dev) for i in range(256)])
+        kb, yb = torch.stack(ks), torch.stack(ys)
+        for _ in range(reps):
+            k1 = kb.detach().requires_grad_(True)
+            mad.gp_log_marginal_batched(k1, yb).sum().backward()
     else:
         raise SystemExit(f"unknown counter workload {name!r}")
     torch.cuda.synchronize()
@@ -348,7 +361,7 @@ def counter_workload(name: str, reps: int):
 
 
 COUNTER_WORKLOADS = ("factor_fp64", "solve_fp64", "mixed_bf16x3", "mixed_bf16x6", "mixed_fp32", "vbatched_fp64",
-                     "vbatched_fp32", "adjoint")
+                     "vbatched_fp32", "adjoint", "adjoint_batch")
 
 
 def main():

@@ -40,8 +40,16 @@ __device__ __forceinline__ int trmm_chunks(int64_t n, int ti, int64_t ck)
 __global__ __launch_bounds__(256, 2) void trmm_part_kernel(const double *__restrict__ Lt, int64_t ldlt,
                                                            const double *__restrict__ Gt, int64_t ldgt, int64_t n,
                                                            int64_t ck, double *__restrict__ Wp, double *__restrict__ P,
-                                                           int64_t ldp, int direct)
+                                                           int64_t ldp, int direct, int64_t sLt, int64_t sGt,
+                                                           int64_t sWp, int64_t sP)
 {
+    {  // batched launches (every kernel of this file): system z = blockIdx.z
+        const int64_t z = blockIdx.z;
+        Lt += z * sLt;
+        Gt += z * sGt;
+        if (Wp) Wp += z * sWp;
+        if (P) P += z * sP;
+    }
     int u = (int)blockIdx.x, ti = 0, nch = trmm_chunks(n, 0, ck);
     while (u >= (ti + 1) * nch) {  // uniform: item -> tile row
         u -= (ti + 1) * nch;
@@ -77,8 +85,10 @@ __global__ __launch_bounds__(256, 2) void trmm_part_kernel(const double *__restr
 
 // P tile (ti, tj) = sum of its partials in chunk order; lower part only on diagonal tiles.
 __global__ __launch_bounds__(256) void trmm_reduce_kernel(const double *__restrict__ Wp, int64_t n, int64_t ck,
-                                                          double *__restrict__ P, int64_t ldp)
+                                                          double *__restrict__ P, int64_t ldp, int64_t sWp, int64_t sP)
 {
+    Wp += (int64_t)blockIdx.z * sWp;
+    P += (int64_t)blockIdx.z * sP;
     int ti, tj;
     lower_tile((int)blockIdx.x, ti, tj);
     int64_t base = 0;
@@ -104,9 +114,12 @@ __global__ __launch_bounds__(256) void trmm_reduce_kernel(const double *__restri
 //   MODE 4: out = tril(in)^T  (out[i][j] = in[j][i] for j >= i, else 0)
 template <int MODE>
 __global__ __launch_bounds__(256) void sym_tile_kernel(const double *__restrict__ in, int64_t ldi, int64_t n,
-                                                       double *__restrict__ out, int64_t ldo)
+                                                       double *__restrict__ out, int64_t ldo, int64_t sIn,
+                                                       int64_t sOut)
 {
     __shared__ double t[32][33];
+    in += (int64_t)blockIdx.z * sIn;
+    out += (int64_t)blockIdx.z * sOut;
     const int64_t bi = blockIdx.x, bj = blockIdx.y;  // output tile rows bi, cols bj
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     // mirrored tile in[bj.., bi..] -> t[x][y] = in[bj*32 + x][bi*32 + y]; each thread issues all
@@ -142,9 +155,13 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(const double *__restrict_
 // U[i][c] = sum_{k >= i} L[k][i] X[k][c] (U = L^T X, L lower): one workgroup per row i.
 __global__ __launch_bounds__(256) void trmv_t_kernel(const double *__restrict__ L, int64_t ldl, int64_t n,
                                                      const double *__restrict__ X, int64_t ldx, int64_t r,
-                                                     double *__restrict__ U, int64_t ldu)
+                                                     double *__restrict__ U, int64_t ldu, int64_t sL, int64_t sX,
+                                                     int64_t sU)
 {
     __shared__ double red[256];
+    L += (int64_t)blockIdx.z * sL;
+    X += (int64_t)blockIdx.z * sX;
+    U += (int64_t)blockIdx.z * sU;
     const int64_t i = blockIdx.x;
     for (int64_t c = 0; c < r; ++c) {
         double s = 0.0;
@@ -165,8 +182,15 @@ __global__ __launch_bounds__(256) void rank2_lower_kernel(double *__restrict__ L
                                                           const double *__restrict__ Bb, int64_t ldbb,
                                                           const double *__restrict__ U, int64_t ldu,
                                                           const double *__restrict__ X, int64_t ldx,
-                                                          const double *__restrict__ V, int64_t ldv, int64_t r)
+                                                          const double *__restrict__ V, int64_t ldv, int64_t r,
+                                                          int64_t sLb, int64_t sBb, int64_t sU, int64_t sX, int64_t sV)
 {
+    const int64_t z = blockIdx.z;
+    Lb += z * sLb;
+    Bb += z * sBb;
+    U += z * sU;
+    X += z * sX;
+    V += z * sV;
     const int64_t j = blockIdx.y;
     const int64_t i = j + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -175,59 +199,75 @@ __global__ __launch_bounds__(256) void rank2_lower_kernel(double *__restrict__ L
     Lb[i + j * ldlb] -= s;
 }
 
-// Lt = tril(L)^T, Gt = tril(G)^T; Wp: n*n doubles of scratch for the partials.
+// out (n x n, ld ldo) = in (ld ldi), one column per workgroup, system z = blockIdx.z.
+__global__ __launch_bounds__(256) void copy_square_kernel(const double *__restrict__ in, int64_t ldi, int64_t n,
+                                                          double *__restrict__ out, int64_t ldo, int64_t sIn,
+                                                          int64_t sOut)
+{
+    const int64_t z = blockIdx.z, j = blockIdx.x;
+    for (int64_t i = threadIdx.x; i < n; i += 256) out[z * sOut + i + j * ldo] = in[z * sIn + i + j * ldi];
+}
+
+// Lt = tril(L)^T, Gt = tril(G)^T; Wp: n*n doubles of scratch for the partials.  count systems,
+// each operand of system z at its base + z times its stride.
 void launch_trmm_tn_lower(const double *Lt, int64_t ldlt, const double *Gt, int64_t ldgt, int64_t n, double *P,
-                          int64_t ldp, double *Wp, hipStream_t s)
+                          int64_t ldp, double *Wp, hipStream_t s, unsigned count, int64_t sLt, int64_t sGt,
+                          int64_t sP, int64_t sWp)
 {
     const int T = (int)((n + 127) / 128);
-    auto count = [&](int64_t ck) {
+    auto items_of = [&](int64_t ck) {
         int64_t items = 0;
         for (int t = 0; t < T; ++t) items += (int64_t)(t + 1) * ((n - 128 * (int64_t)t + ck - 1) / ck);
         return items;
     };
     int C = (T + 2) / 3 > 1 ? (T + 2) / 3 : 1;  // chunk of 128 C >= n/3
-    int64_t items = count(128 * (int64_t)C);
-    while (items * 128 * 128 > n * n && C < T) items = count(128 * (int64_t)(++C));
+    int64_t items = items_of(128 * (int64_t)C);
+    while (items * 128 * 128 > n * n && C < T) items = items_of(128 * (int64_t)(++C));
     const int direct = items * 128 * 128 > n * n ? 1 : 0;  // then C == T: one chunk per tile
     const int64_t ck = 128 * (int64_t)C;
     if (direct) {
         // P may be Gt's buffer, still read by other workgroups of the launch: the tiles go to
         // Wp (as an n x n matrix) and are copied over afterwards (small n only)
-        hipLaunchKernelGGL(trmm_part_kernel, dim3((unsigned)items), dim3(256), 0, s, Lt, ldlt, Gt, ldgt, n, ck,
-                           (double *)nullptr, Wp, n, 1);
-        (void)hipMemcpy2DAsync(P, (size_t)ldp * sizeof(double), Wp, (size_t)n * sizeof(double),
-                               (size_t)n * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, s);
+        hipLaunchKernelGGL(trmm_part_kernel, dim3((unsigned)items, 1, count), dim3(256), 0, s, Lt, ldlt, Gt, ldgt, n,
+                           ck, (double *)nullptr, Wp, n, 1, sLt, sGt, (int64_t)0, sWp);
+        hipLaunchKernelGGL(copy_square_kernel, dim3((unsigned)n, 1, count), dim3(256), 0, s, Wp, n, n, P, ldp, sWp,
+                           sP);
     } else {
-        hipLaunchKernelGGL(trmm_part_kernel, dim3((unsigned)items), dim3(256), 0, s, Lt, ldlt, Gt, ldgt, n, ck, Wp,
-                           (double *)nullptr, (int64_t)0, 0);
-        hipLaunchKernelGGL(trmm_reduce_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, Wp, n, ck, P,
-                           ldp);
+        hipLaunchKernelGGL(trmm_part_kernel, dim3((unsigned)items, 1, count), dim3(256), 0, s, Lt, ldlt, Gt, ldgt, n,
+                           ck, Wp, (double *)nullptr, (int64_t)0, 0, sLt, sGt, sWp, (int64_t)0);
+        hipLaunchKernelGGL(trmm_reduce_kernel, dim3((unsigned)(T * (T + 1) / 2), 1, count), dim3(256), 0, s, Wp, n,
+                           ck, P, ldp, sWp, sP);
     }
 }
 
-void launch_sym_tile(int mode, const double *in, int64_t ldi, int64_t n, double *out, int64_t ldo, hipStream_t s)
+void launch_sym_tile(int mode, const double *in, int64_t ldi, int64_t n, double *out, int64_t ldo, hipStream_t s,
+                     unsigned count, int64_t sIn, int64_t sOut)
 {
-    const dim3 g((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
+    const dim3 g((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32), count);
     if (mode == 0)
-        hipLaunchKernelGGL(sym_tile_kernel<0>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
+        hipLaunchKernelGGL(sym_tile_kernel<0>, g, dim3(256), 0, s, in, ldi, n, out, ldo, sIn, sOut);
     else if (mode == 1)
-        hipLaunchKernelGGL(sym_tile_kernel<1>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
+        hipLaunchKernelGGL(sym_tile_kernel<1>, g, dim3(256), 0, s, in, ldi, n, out, ldo, sIn, sOut);
     else if (mode == 2)
-        hipLaunchKernelGGL(sym_tile_kernel<2>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
+        hipLaunchKernelGGL(sym_tile_kernel<2>, g, dim3(256), 0, s, in, ldi, n, out, ldo, sIn, sOut);
     else if (mode == 3)
-        hipLaunchKernelGGL(sym_tile_kernel<3>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
+        hipLaunchKernelGGL(sym_tile_kernel<3>, g, dim3(256), 0, s, in, ldi, n, out, ldo, sIn, sOut);
     else
-        hipLaunchKernelGGL(sym_tile_kernel<4>, g, dim3(256), 0, s, in, ldi, n, out, ldo);
+        hipLaunchKernelGGL(sym_tile_kernel<4>, g, dim3(256), 0, s, in, ldi, n, out, ldo, sIn, sOut);
 }
 
 void launch_solve_adjoint_update(const double *L, int64_t ldl, int64_t n, int64_t r, const double *X, int64_t ldx,
                                  const double *Bb, int64_t ldbb, double *Lb, int64_t ldlb, double *U, double *V,
-                                 hipStream_t s)
+                                 hipStream_t s, unsigned count, int64_t sL, int64_t sX, int64_t sBb, int64_t sLb,
+                                 int64_t sUV)
 {
-    hipLaunchKernelGGL(trmv_t_kernel, dim3((unsigned)n), dim3(256), 0, s, L, ldl, n, X, ldx, r, U, n);
-    hipLaunchKernelGGL(trmv_t_kernel, dim3((unsigned)n), dim3(256), 0, s, L, ldl, n, Bb, ldbb, r, V, n);
-    const dim3 g((unsigned)((n + 255) / 256), (unsigned)n);
-    hipLaunchKernelGGL(rank2_lower_kernel, g, dim3(256), 0, s, Lb, ldlb, n, Bb, ldbb, U, n, X, ldx, V, n, r);
+    hipLaunchKernelGGL(trmv_t_kernel, dim3((unsigned)n, 1, count), dim3(256), 0, s, L, ldl, n, X, ldx, r, U, n, sL, sX,
+                       sUV);
+    hipLaunchKernelGGL(trmv_t_kernel, dim3((unsigned)n, 1, count), dim3(256), 0, s, L, ldl, n, Bb, ldbb, r, V, n, sL,
+                       sBb, sUV);
+    const dim3 g((unsigned)((n + 255) / 256), (unsigned)n, count);
+    hipLaunchKernelGGL(rank2_lower_kernel, g, dim3(256), 0, s, Lb, ldlb, n, Bb, ldbb, U, n, X, ldx, V, n, r, sLb, sBb,
+                       sUV, sX, sUV);
 }
 
 }  // namespace mi

@@ -188,7 +188,7 @@ int factor_lower(mi_llt_handle h, T *A, int64_t lda, int64_t n, double *wleaf, i
     int rc = reserve_sync(h, 0);
     if (rc) return rc;
     hipStream_t s = h->stream;
-    mi::enqueue_factorization<T>(s, A, lda, n, wleaf, d_info, h->sync, prec, wsplit,
+    mi::enqueue_factorization<T>(s, A, lda, n, wleaf, d_info, h->sync, (int *)h->sync + mi::kSyncStatus, prec, wsplit,
                                  [&](double fl, auto &&launch) { traced(h, s, MI_TRACE_POTRF, fl, launch); });
     return 0;
 }
@@ -1139,6 +1139,207 @@ int mi_dpotrs_adjoint(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const
     if ((rc = solve_only<double>(h, dL, ldl, n, dXbar, ldxb, nrhs, winv, nullptr))) return rc;  // Bbar
     mi::launch_solve_adjoint_update(dL, ldl, n, nrhs, dX, ldx, dXbar, ldxb, dLbar, ldlb, U, V, s);
     return check_launch(h, "potrs_adjoint");
+}
+
+}  // extern "C"
+
+// ---- batched, uniform order -----------------------------------------------------------------
+// `batch` systems of one order n, system z at dA + z * strideA (likewise every other array): the
+// same kernels as the single-system entry points, each launch covering every system
+// (blockIdx.z), so that a batch of small systems fills the GPU.  Per system, the workspace holds
+// its own panel images, block inverses, packed solve copies and adjoint scratch, and the sync
+// words its own ticket counters and panel flag (the status word is shared).
+namespace {
+
+// An n x cols array per system at p + z * stride, leading dimension ld.
+int check_batch_args(int64_t n, int64_t cols, int64_t batch, int64_t ld, int64_t stride, const void *p)
+{
+    if (batch < 0 || batch > 65535) return 1;
+    if (n > 0 && cols > 0 && batch > 0 && p == nullptr) return 2;
+    if (ld < (n > 1 ? n : 1)) return 3;
+    if (batch > 1 && stride < ld * cols) return 4;  // systems must not overlap
+    return 0;
+}
+
+struct BatchLayout {
+    double *leaf = nullptr, *inv = nullptr, *pack = nullptr, *aux = nullptr;
+    mi::Batch bt;
+};
+
+// Workspace [ leaf x batch | inv x batch | pack x batch | aux x batch ] and the per-system sync
+// blocks (after the handle's own); strides in elements.
+int prepare_batch(mi_llt_handle h, int64_t n, int64_t nrhs, int64_t batch, bool inv, size_t aux_doubles,
+                  BatchLayout &bl)
+{
+    const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
+    const size_t leaf_d = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double)) / sizeof(double);
+    const size_t inv_d = inv ? round_up((size_t)nblk * mi::kInvDoubles * sizeof(double)) / sizeof(double) : 0;
+    const size_t pack_d = nrhs > 0 ? round_up(2 * (size_t)mi::solve_pack_doubles(n, nrhs) * sizeof(double)) / sizeof(double) : 0;
+    const size_t aux_d = round_up(aux_doubles * sizeof(double)) / sizeof(double);
+    const size_t per = leaf_d + inv_d + pack_d + aux_d;
+    int rc = reserve(h, per * (size_t)batch * sizeof(double));
+    if (rc) return rc;
+    rc = reserve_sync(h, (size_t)batch * mi::kSyncWords);
+    if (rc) return rc;
+    double *p = (double *)h->ws;
+    bl.leaf = p;
+    bl.inv = p + leaf_d * batch;
+    bl.pack = bl.inv + inv_d * batch;
+    bl.aux = bl.pack + pack_d * batch;
+    bl.bt.count = (unsigned)batch;
+    bl.bt.leaf = (int64_t)leaf_d;
+    bl.bt.inv = (int64_t)inv_d;
+    bl.bt.pack = (int64_t)pack_d;
+    bl.bt.aux = (int64_t)aux_d;
+    bl.bt.sync = mi::kSyncWords;
+    return 0;
+}
+
+uint32_t *batch_sync(mi_llt_handle h) { return h->sync + mi::kSyncWords; }  // system 0's block
+int *shared_status(mi_llt_handle h) { return (int *)h->sync + mi::kSyncStatus; }
+
+__global__ void set_ints_kernel(int *p, int count, int v)
+{
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < count; i += (int)(gridDim.x * blockDim.x)) p[i] = v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mi_dpotrf_batched_async(mi_llt_handle h, char uplo, int64_t n, int64_t batch, double *dA, int64_t lda,
+                            int64_t strideA, int *d_info)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (n < 0) return -3;
+    if (batch < 0 || batch > 65535) return -4;
+    if (int e = check_batch_args(n, n, batch, lda, strideA, dA)) return e == 2 ? -5 : (e == 3 ? -6 : -7);
+    if (d_info == nullptr && batch > 0) return -8;
+    if (batch == 0) return 0;
+    hipStream_t s = h->stream;
+    hipLaunchKernelGGL(set_ints_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, s, d_info, (int)batch, 0);
+    if (n == 0) return check_launch(h, "set_info");
+    BatchLayout bl;
+    int rc = prepare_batch(h, n, 0, batch, false, 0, bl);
+    if (rc) return rc;
+    bl.bt.a = strideA;
+    mi::enqueue_factorization<double>(
+        s, dA, lda, n, bl.leaf, d_info, batch_sync(h), shared_status(h), MI_PREC_FP64, nullptr,
+        [&](double fl, auto &&launch) { traced(h, s, MI_TRACE_POTRF, fl * (double)batch, launch); }, bl.bt);
+    return check_launch(h, "potrf_batched");
+}
+
+int mi_dpotrs_batched(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, int64_t batch, const double *dL,
+                      int64_t ldl, int64_t strideL, double *dB, int64_t ldb, int64_t strideB)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (batch < 0 || batch > 65535) return -5;
+    if (int e = check_batch_args(n, n, batch, ldl, strideL, dL)) return e == 2 ? -6 : (e == 3 ? -7 : -8);
+    if (nrhs > 0) {
+        if (int e = check_batch_args(n, nrhs, batch, ldb, strideB, dB)) return e == 2 ? -9 : (e == 3 ? -10 : -11);
+    }
+    if (n == 0 || nrhs == 0 || batch == 0) return 0;
+    BatchLayout bl;
+    int rc = prepare_batch(h, n, nrhs, batch, true, 0, bl);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    bl.bt.a = strideL;
+    bl.bt.b = strideB;
+    mi::launch_leaf_inverses<double>(dL, ldl, n, bl.leaf, s, bl.bt);
+    traced(h, s, MI_TRACE_TRTRI, (double)n * mi::kNB * mi::kNB / 3.0 * (double)batch,
+           [&] { mi::launch_block_inverses<double>(dL, ldl, n, bl.leaf, bl.inv, nullptr, s, true, bl.bt); });
+    traced(h, s, MI_TRACE_SOLVE, 2.0 * (double)n * (double)n * (double)nrhs * (double)batch, [&] {
+        mi::launch_solve_persistent<double>(dL, ldl, n, dB, ldb, nrhs, bl.inv, bl.pack,
+                                            (int *)batch_sync(h) + mi::kSyncSolveTickets, nullptr, shared_status(h), s,
+                                            3, false, bl.bt);
+    });
+    return check_launch(h, "potrs_batched");
+}
+
+int mi_dpotrf_adjoint_batched(mi_llt_handle h, char uplo, int64_t n, int64_t batch, const double *dL, int64_t ldl,
+                              int64_t strideL, double *dG, int64_t ldg, int64_t strideG)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (n < 0) return -3;
+    if (batch < 0 || batch > 65535) return -4;
+    if (int e = check_batch_args(n, n, batch, ldl, strideL, dL)) return e == 2 ? -5 : (e == 3 ? -6 : -7);
+    if (int e = check_batch_args(n, n, batch, ldg, strideG, dG)) return e == 2 ? -8 : (e == 3 ? -9 : -10);
+    if (n == 0 || batch == 0) return 0;
+    const size_t sq = (size_t)n * (size_t)n;
+    BatchLayout bl;
+    int rc = prepare_batch(h, n, 0, batch, true, 3 * sq, bl);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    const unsigned nb = (unsigned)batch;
+    const int64_t sa = bl.bt.aux;  // per-system block of 3 n^2: W2 | W3 | Wt
+    double *W2 = bl.aux, *W3 = bl.aux + sq, *Wt = bl.aux + 2 * sq;
+    const double nd = (double)n, bd = (double)batch;
+    bl.bt.a = strideL;
+    mi::launch_sym_tile(4, dL, ldl, n, Wt, n, s, nb, strideL, sa);  // tril(L)^T
+    mi::launch_sym_tile(4, dG, ldg, n, W3, n, s, nb, strideG, sa);  // tril(Lbar)^T, then P
+    traced(h, s, MI_TRACE_GEMM, nd * nd * nd / 3.0 * bd,
+           [&] { mi::launch_trmm_tn_lower(Wt, n, W3, n, n, W3, n, W2, s, nb, sa, sa, sa, sa); });
+    mi::launch_sym_tile(0, W3, n, n, W2, n, s, nb, sa, sa);  // M
+    mi::launch_leaf_inverses<double>(dL, ldl, n, bl.leaf, s, bl.bt);
+    traced(h, s, MI_TRACE_TRTRI, nd * mi::kNB * mi::kNB / 3.0 * bd,
+           [&] { mi::launch_block_inverses<double>(dL, ldl, n, bl.leaf, bl.inv, nullptr, s, false, bl.bt); });
+    traced(h, s, MI_TRACE_SOLVE, nd * nd * nd * bd,
+           [&] { mi::launch_trsm_xl_multi(Wt, n, n, W2, n, n, bl.inv, false, s, nb, sa, sa, bl.bt.inv); });
+    mi::launch_sym_tile(1, W2, n, n, W3, n, s, nb, sa, sa);  // Y^T
+    traced(h, s, MI_TRACE_SOLVE, nd * nd * nd / 2.0 * bd,
+           [&] { mi::launch_trsm_xl_multi(Wt, n, n, W3, n, n, bl.inv, true, s, nb, sa, sa, bl.bt.inv); });
+    mi::launch_sym_tile(3, W3, n, n, dG, ldg, s, nb, sa, strideG);  // Abar = sym(S)
+    return check_launch(h, "potrf_adjoint_batched");
+}
+
+int mi_dpotrs_adjoint_batched(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, int64_t batch, const double *dL,
+                              int64_t ldl, int64_t strideL, const double *dX, int64_t ldx, int64_t strideX,
+                              double *dXbar, int64_t ldxb, int64_t strideXbar, double *dLbar, int64_t ldlb,
+                              int64_t strideLbar)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (batch < 0 || batch > 65535) return -5;
+    if (int e = check_batch_args(n, n, batch, ldl, strideL, dL)) return e == 2 ? -6 : (e == 3 ? -7 : -8);
+    if (nrhs > 0) {
+        if (int e = check_batch_args(n, nrhs, batch, ldx, strideX, dX)) return e == 2 ? -9 : (e == 3 ? -10 : -11);
+        if (int e = check_batch_args(n, nrhs, batch, ldxb, strideXbar, dXbar)) return e == 2 ? -12 : (e == 3 ? -13 : -14);
+    }
+    if (int e = check_batch_args(n, n, batch, ldlb, strideLbar, dLbar)) return e == 2 ? -15 : (e == 3 ? -16 : -17);
+    if (n == 0 || nrhs == 0 || batch == 0) return 0;
+    const size_t vec = (size_t)n * (size_t)nrhs;
+    BatchLayout bl;
+    int rc = prepare_batch(h, n, nrhs, batch, true, 2 * vec, bl);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    bl.bt.a = strideL;
+    bl.bt.b = strideXbar;
+    mi::launch_leaf_inverses<double>(dL, ldl, n, bl.leaf, s, bl.bt);
+    mi::launch_block_inverses<double>(dL, ldl, n, bl.leaf, bl.inv, nullptr, s, true, bl.bt);
+    traced(h, s, MI_TRACE_SOLVE, 2.0 * (double)n * (double)n * (double)nrhs * (double)batch, [&] {  // Bbar
+        mi::launch_solve_persistent<double>(dL, ldl, n, dXbar, ldxb, nrhs, bl.inv, bl.pack,
+                                            (int *)batch_sync(h) + mi::kSyncSolveTickets, nullptr, shared_status(h), s,
+                                            3, false, bl.bt);
+    });
+    double *U = bl.aux, *V = bl.aux + vec;
+    mi::launch_solve_adjoint_update(dL, ldl, n, nrhs, dX, ldx, dXbar, ldxb, dLbar, ldlb, U, V, s, (unsigned)batch,
+                                    strideL, strideX, strideXbar, strideLbar, bl.bt.aux);
+    return check_launch(h, "potrs_adjoint_batched");
 }
 
 }  // extern "C"

@@ -17,7 +17,7 @@ namespace mi {
 // Zeroes the ticket counter and the panel flag (sync[kSyncFactorTickets], sync[kSyncFlag]) on
 // the stream: a kernel, not a memset, so that a captured factorization replays it
 // (factor_step.hip).
-void launch_factor_reset(uint32_t *sync, hipStream_t s);
+void launch_factor_reset(uint32_t *sync, hipStream_t s, const Batch &bt = Batch());
 
 // Algorithmic flop counts (LAPACK convention): POTRF of b: b^3/3 + b^2/2 + b/6; TRSM m x b
 // panel: m b^2; panel update by the previous panel: 2 (n - k0) b NB minus its upper part.
@@ -30,17 +30,20 @@ inline double panel_flops(int64_t n, int64_t k0, int64_t NB)
 }
 
 // Queues the factorization of the lower triangle of A (n x n, ld lda) on stream s.
-// sync: the synchronisation words (status, tickets, panel flag at kSync*); wleaf: the panel
-// images (kPanelDoubles per 128-block).  each(flops, launch) runs every step's launch (the
-// tracing hook of the handle API); the ticket bases and flag epochs count from the reset, so
-// every call and every replay of a captured call has the same accounting.
+// sync: the synchronisation words whose ticket counter and panel flag the factorization uses
+// (at kSyncFactorTickets, kSyncFlag); status: the status word; wleaf: the panel images
+// (kPanelDoubles per 128-block).  each(flops, launch) runs every step's launch (the tracing
+// hook of the handle API); the ticket bases and flag epochs count from the reset, so every
+// call and every replay of a captured call has the same accounting.  bt: a batch of systems of
+// the same order (each launch covers all of them; every system has its own block of sync
+// words, bt.sync apart, and its info at d_info + z).
 template <typename T, class Each>
 void enqueue_factorization(hipStream_t s, T *A, int64_t lda, int64_t n, double *wleaf, int *d_info, uint32_t *sync,
-                           int prec, __bf16 *wsplit, Each &&each)
+                           int *status, int prec, __bf16 *wsplit, Each &&each, const Batch &bt = Batch())
 {
     constexpr int64_t NB = kNB;
     const int64_t nblk = (n + NB - 1) / NB;
-    launch_factor_reset(sync, s);
+    launch_factor_reset(sync, s, bt);
     uint32_t tickets = 0, epoch_now = 0;
     for (int64_t k = -1; k < nblk; ++k) {
         const int64_t kp = (k + 1 < nblk) ? (k + 1) * NB : -1;  // panel of this launch
@@ -58,7 +61,7 @@ void enqueue_factorization(hipStream_t s, T *A, int64_t lda, int64_t n, double *
         uint32_t grid = 0;
         each(fl, [&] {
             launch_factor_step<T>(A, lda, n, kp, k, wleaf, d_info, sync + kSyncFactorTickets, tickets, sync + kSyncFlag,
-                                  epoch, (int *)sync + kSyncStatus, s, &grid, prec, wsplit);
+                                  epoch, status, s, &grid, prec, wsplit, bt);
         });
         tickets += grid;
     }

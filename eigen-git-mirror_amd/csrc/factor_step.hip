@@ -57,9 +57,18 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
                                                           int Gp, int c0, int nsyrk, int64_t slab,
                                                           double *__restrict__ Wleaf, int *info, uint32_t *tickets,
                                                           uint32_t ticket_base, uint32_t *flag, uint32_t epoch,
-                                                          int *status, __bf16 *Wsplit, int64_t slot_elems)
+                                                          int *status, __bf16 *Wsplit, int64_t slot_elems,
+                                                          int64_t sA, int64_t sW, int64_t sSync)
 {
     __shared__ __align__(16) double smem[kSmem];
+    {  // batched launches: system z = blockIdx.z (its matrix, panel images, tickets, flag, info)
+        const int64_t z = blockIdx.z;
+        A += z * sA;
+        Wleaf += z * sW;
+        tickets += z * sSync;
+        flag += z * sSync;
+        info += z;
+    }
     __shared__ int s_role, s_ok, s_stop;
     // Every workgroup draws its ticket, even when it then stops: the host counts them.  The
     // failure words (info, status) are read in the same round trip as the ticket, so a
@@ -108,15 +117,16 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
                           0);
 }
 
-__global__ void factor_reset_kernel(uint32_t *sync)
+__global__ void factor_reset_kernel(uint32_t *sync, int64_t sSync)
 {
+    sync += (int64_t)blockIdx.z * sSync;
     if (threadIdx.x == 0) sync[kSyncFactorTickets] = 0u;
     if (threadIdx.x == 1) sync[kSyncFlag] = 0u;
 }
 
-void launch_factor_reset(uint32_t *sync, hipStream_t s)
+void launch_factor_reset(uint32_t *sync, hipStream_t s, const Batch &bt)
 {
-    hipLaunchKernelGGL(factor_reset_kernel, dim3(1), dim3(64), 0, s, sync);
+    hipLaunchKernelGGL(factor_reset_kernel, dim3(1, 1, bt.count), dim3(64), 0, s, sync, bt.sync);
 }
 
 // Trailing-update tiles of launch k (panel k just finished; k = -1: none).
@@ -134,20 +144,23 @@ int pair_tiles(int64_t n, int64_t k, int *c0_out)
 template <typename S, int PREC>
 static void launch_prec(dim3 grid, hipStream_t s, int K, S *A, int64_t lda, int64_t n, int64_t kp, int Gp, int c0,
                         int nsyrk, int64_t slab, double *Wleaf, int *info, uint32_t *tickets, uint32_t ticket_base,
-                        uint32_t *flag, uint32_t epoch, int *status, __bf16 *Wsplit, int64_t slot_elems)
+                        uint32_t *flag, uint32_t epoch, int *status, __bf16 *Wsplit, int64_t slot_elems,
+                        const Batch &bt)
 {
     if (K == 256)
         hipLaunchKernelGGL((factor_step_kernel<S, PREC, 256>), grid, dim3(256), 0, s, A, lda, n, kp, Gp, c0, nsyrk,
-                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status, Wsplit, slot_elems);
+                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status, Wsplit, slot_elems, bt.a,
+                           bt.leaf, bt.sync);
     else
         hipLaunchKernelGGL((factor_step_kernel<S, PREC, 128>), grid, dim3(256), 0, s, A, lda, n, kp, Gp, c0, nsyrk,
-                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status, Wsplit, slot_elems);
+                           slab, Wleaf, info, tickets, ticket_base, flag, epoch, status, Wsplit, slot_elems, bt.a,
+                           bt.leaf, bt.sync);
 }
 
 template <typename S>
 void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
-                        hipStream_t s, uint32_t *grid_out, int prec, __bf16 *Wsplit)
+                        hipStream_t s, uint32_t *grid_out, int prec, __bf16 *Wsplit, const Batch &bt)
 {
     const int Gp = kp >= 0 ? (int)((n - kp + 127) / 128) : 0;
     int c0 = 0;
@@ -157,31 +170,31 @@ void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, dou
     const unsigned grid = (unsigned)(Gp + nsyrk);
     *grid_out = grid;
     if (grid == 0) return;
-    const dim3 g(grid);
+    const dim3 g(grid, 1, bt.count);
     const int64_t slot = split_slot_elems(n, prec == kPrecBF16x6 ? 3 : 2);
     if constexpr (sizeof(S) == 8) {
         launch_prec<S, kPrecF64>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base, flag,
-                                 epoch, status, nullptr, slot);
+                                 epoch, status, nullptr, slot, bt);
     } else if (prec == kPrecBF16) {
         launch_prec<S, kPrecBF16>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
-                                  flag, epoch, status, nullptr, slot);
+                                  flag, epoch, status, nullptr, slot, bt);
     } else if (prec == kPrecBF16x6) {
         launch_prec<S, kPrecBF16x6>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
-                                    flag, epoch, status, Wsplit, slot);
+                                    flag, epoch, status, Wsplit, slot, bt);
     } else if (prec == kPrecBF16x3) {
         launch_prec<S, kPrecBF16x3>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base,
-                                    flag, epoch, status, Wsplit, slot);
+                                    flag, epoch, status, Wsplit, slot, bt);
     } else {
         launch_prec<S, kPrecF32>(g, s, K, A, lda, n, kp, Gp, c0, nsyrk, slab, Wleaf, info, tickets, ticket_base, flag,
-                                 epoch, status, nullptr, slot);
+                                 epoch, status, nullptr, slot, bt);
     }
 }
 
 template void launch_factor_step<double>(double *, int64_t, int64_t, int64_t, int64_t, double *, int *, uint32_t *,
                                          uint32_t, uint32_t *, uint32_t, int *, hipStream_t, uint32_t *, int,
-                                         __bf16 *);
+                                         __bf16 *, const Batch &);
 template void launch_factor_step<float>(float *, int64_t, int64_t, int64_t, int64_t, double *, int *, uint32_t *,
                                         uint32_t, uint32_t *, uint32_t, int *, hipStream_t, uint32_t *, int,
-                                         __bf16 *);
+                                        __bf16 *, const Batch &);
 
 }  // namespace mi

@@ -25,6 +25,16 @@ constexpr int64_t kInvDoubles = 6 * 128 * 128;
 constexpr int64_t kInvX = 128 * 128, kInvM = 2 * 128 * 128, kInvN = 3 * 128 * 128;  // offsets
 constexpr int64_t kInvM2 = 4 * 128 * 128, kInvN2 = 5 * 128 * 128;
 
+// A batched launch: `count` systems of one order, system z (blockIdx.z) at every base pointer
+// plus z times that buffer's stride in elements -- a: the matrix or factor, b: right-hand
+// sides, leaf: panel images, inv: block inverses, pack: the solve's packed copies, sync: the
+// system's block of synchronisation words, g / aux / aux2: the adjoint's operands and scratch;
+// an info pointer advances by z.  The default is one system (the strides are then unused).
+struct Batch {
+    unsigned count = 1;
+    int64_t a = 0, b = 0, leaf = 0, inv = 0, pack = 0, sync = 0, g = 0, aux = 0, aux2 = 0;
+};
+
 // Synchronisation words of a handle (32-bit words in device memory).  Each group sits on its
 // own 128-byte line: every starting workgroup of a factorization step takes an atomic on the
 // ticket counter and the waiting ones poll the flag, and a read of another word of that line
@@ -53,7 +63,8 @@ constexpr int MI_FACTOR_TICKET = 4;  // a factorization workgroup drew a ticket 
 template <typename S>
 void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
-                        hipStream_t s, uint32_t *grid_out, int prec, __bf16 *Wsplit = nullptr);
+                        hipStream_t s, uint32_t *grid_out, int prec, __bf16 *Wsplit = nullptr,
+                        const Batch &bt = Batch());
 // BF16 elements of one part (n rows rounded up to 128, x 128 columns) and of one slot (2 parts
 // for kPrecBF16x3, 3 for kPrecBF16x6) of the split-panel ring (Wsplit, 4 slots).
 __host__ __device__ inline int64_t split_part_elems(int64_t n) { return ((n + 127) / 128 * 128) * 128; }
@@ -62,10 +73,11 @@ __host__ __device__ inline int64_t split_slot_elems(int64_t n, int parts) { retu
 // couplings = false: X_b^T and X_b only (for users of Winv other than the solve).
 template <typename T>
 void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv, const int *info,
-                           hipStream_t s, bool couplings = true);
+                           hipStream_t s, bool couplings = true, const Batch &bt = Batch());
 // Wleaf from an already factored matrix (standalone solve).
 template <typename T>
-void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s);
+void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s,
+                          const Batch &bt = Batch());
 // Solve: forward then backward sweep in place on B (one persistent launch each, each after a
 // small reset launch).  tickets: 2 ints; Ypk: 2 * solve_pack_doubles(n, nrhs) doubles of
 // scratch (each finished block's packed copy for its consumers, per sweep).
@@ -75,7 +87,7 @@ template <typename T>
 // sweep then reads with the forward sweep's coalesced access.
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, double *Ypk, int *tickets, const int *info, int *status, hipStream_t s,
-                             int sweeps = 3, bool upper_copy = false);
+                             int sweeps = 3, bool upper_copy = false, const Batch &bt = Batch());
 // Doubles of one sweep's packed block copies for n rows and nrhs right-hand sides.
 inline int64_t solve_pack_doubles(int64_t n, int64_t nrhs) { return ((n + 127) / 128) * ((nrhs + 15) / 16) * 2048; }
 // Symmetric-storage helpers.
@@ -142,16 +154,20 @@ void launch_rank_update(double *L, int64_t ldl, int64_t n, const double *v, int6
 // P (lower) = tril(L^T G) for lower L, G, from Lt = tril(L)^T and Gt = tril(G)^T; Wp: n*n
 // doubles of scratch (the k-chunk partials).  P may be Gt's buffer.
 void launch_trmm_tn_lower(const double *Lt, int64_t ldlt, const double *Gt, int64_t ldgt, int64_t n, double *P,
-                          int64_t ldp, double *Wp, hipStream_t s);
+                          int64_t ldp, double *Wp, hipStream_t s, unsigned count = 1, int64_t sLt = 0, int64_t sGt = 0,
+                          int64_t sP = 0, int64_t sWp = 0);
 // mode 0: out = symmetric completion of tril(in), halved; 1: out = in^T; 2: out = (in + in^T)/2.
-void launch_sym_tile(int mode, const double *in, int64_t ldi, int64_t n, double *out, int64_t ldo, hipStream_t s);
+void launch_sym_tile(int mode, const double *in, int64_t ldi, int64_t n, double *out, int64_t ldo, hipStream_t s,
+                     unsigned count = 1, int64_t sIn = 0, int64_t sOut = 0);
 // B (nrow x n) <- B L^{-1} by blocked right-side substitution, one launch per 128-column
 // block, with Lt = L^T; upper: only the block-upper tiles (nrow == n, the result symmetric).
 void launch_trsm_xl_multi(const double *Lt, int64_t ldlt, int64_t n, double *B, int64_t ldb, int64_t nrow,
-                          const double *Winv, bool upper, hipStream_t s);
+                          const double *Winv, bool upper, hipStream_t s, unsigned count = 1, int64_t sLt = 0,
+                          int64_t sB = 0, int64_t sWinv = 0);
 // Lbar (lower) -= tril(Bb (L^T X)^T + X (L^T Bb)^T); U, V: n x r scratch.
 void launch_solve_adjoint_update(const double *L, int64_t ldl, int64_t n, int64_t r, const double *X, int64_t ldx,
                                  const double *Bb, int64_t ldbb, double *Lb, int64_t ldlb, double *U, double *V,
-                                 hipStream_t s);
+                                 hipStream_t s, unsigned count = 1, int64_t sL = 0, int64_t sX = 0, int64_t sBb = 0,
+                                 int64_t sLb = 0, int64_t sUV = 0);
 
 }  // namespace mi

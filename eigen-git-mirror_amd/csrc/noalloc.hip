@@ -92,7 +92,8 @@ extern "C" int mi_dposv_noalloc(char uplo, int64_t n, int64_t nrhs, double *dA, 
 
     hipLaunchKernelGGL(noalloc_begin_kernel, dim3(1), dim3(mi::kSyncWords), 0, s, sync, d_info);
     if (n > 0) {
-        mi::enqueue_factorization<double>(s, dA, lda, n, wleaf, d_info, sync, MI_PREC_FP64, nullptr,
+        mi::enqueue_factorization<double>(s, dA, lda, n, wleaf, d_info, sync, (int *)sync + mi::kSyncStatus,
+                                          MI_PREC_FP64, nullptr,
                                           [](double, auto &&launch) { launch(); });
         if (nrhs > 0) {
             mi::launch_block_inverses<double>(dA, lda, n, wleaf, winv, d_info, s);

@@ -20,12 +20,20 @@
 namespace mi {
 
 // Workgroup b: X_b^T = L_bb^{-T} to Wout + b*kInvDoubles (ld 128) and X_b at + kInvX.
+// Batched launches (every kernel of this file): system z = blockIdx.z, its pointers advanced
+// by z times their strides (s*), its info at info + z.
 template <typename T>
 __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict__ L, int64_t ldl, int64_t n,
                                                             const double *__restrict__ Wleaf,
-                                                            double *__restrict__ Wout, const int *info)
+                                                            double *__restrict__ Wout, const int *info, int64_t sL,
+                                                            int64_t sWl, int64_t sWo)
 {
     __shared__ double smem[kSmem];
+    const int64_t z = blockIdx.z;
+    L += z * sL;
+    Wleaf += z * sWl;
+    Wout += z * sWo;
+    if (info) info += z;
     if (factor_failed(info)) return;
     const int b = (int)blockIdx.x;
     const int64_t kb = (int64_t)b * 128;
@@ -44,8 +52,13 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict_
 // MFMA tiles), depth 128.
 template <typename T>
 __global__ __launch_bounds__(256) void block_coupling_kernel(const T *__restrict__ L, int64_t ldl, int64_t n,
-                                                             double *__restrict__ Winv, const int *info)
+                                                             double *__restrict__ Winv, const int *info, int64_t sL,
+                                                             int64_t sWo)
 {
+    const int64_t z = blockIdx.z;
+    L += z * sL;
+    Winv += z * sWo;
+    if (info) info += z;
     if (factor_failed(info)) return;
     const int nblk = (int)((n + 127) / 128);
     const int q = (int)blockIdx.x / nblk, b = (int)blockIdx.x % nblk;
@@ -99,8 +112,10 @@ __global__ __launch_bounds__(256) void block_coupling_kernel(const T *__restrict
 // inverts leaves 2w and 2w+1 of block b (lane i < 16 holds row i of the leaf).
 template <typename T>
 __global__ __launch_bounds__(256) void leaf_inverse_kernel(const T *__restrict__ L, int64_t lda, int64_t n,
-                                                           double *__restrict__ Wleaf)
+                                                           double *__restrict__ Wleaf, int64_t sL, int64_t sWl)
 {
+    L += (int64_t)blockIdx.z * sL;
+    Wleaf += (int64_t)blockIdx.z * sWl;
     const int l = lane_id(), w = threadIdx.x >> 6;
     const int64_t kb = (int64_t)blockIdx.x * 128;
     const int bs = (int)((n - kb) < 128 ? (n - kb) : 128);
@@ -142,26 +157,29 @@ __global__ __launch_bounds__(256) void leaf_inverse_kernel(const T *__restrict__
 
 template <typename T>
 void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wleaf, double *Winv, const int *info,
-                           hipStream_t s, bool couplings)
+                           hipStream_t s, bool couplings, const Batch &bt)
 {
     const unsigned nblk = (unsigned)((n + 127) / 128);
-    hipLaunchKernelGGL(block_inverse_kernel<T>, dim3(nblk), dim3(256), 0, s, L, lda, n, Wleaf, Winv, info);
+    hipLaunchKernelGGL(block_inverse_kernel<T>, dim3(nblk, 1, bt.count), dim3(256), 0, s, L, lda, n, Wleaf, Winv,
+                       info, bt.a, bt.leaf, bt.inv);
     if (couplings && nblk > 1)
-        hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(4 * nblk), dim3(256), 0, s, L, lda, n, Winv, info);
+        hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(4 * nblk, 1, bt.count), dim3(256), 0, s, L, lda, n, Winv,
+                           info, bt.a, bt.inv);
 }
 
 template <typename T>
-void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s)
+void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s, const Batch &bt)
 {
     const unsigned nblk = (unsigned)((n + 127) / 128);
-    hipLaunchKernelGGL(leaf_inverse_kernel<T>, dim3(nblk), dim3(256), 0, s, L, lda, n, Wleaf);
+    hipLaunchKernelGGL(leaf_inverse_kernel<T>, dim3(nblk, 1, bt.count), dim3(256), 0, s, L, lda, n, Wleaf, bt.a,
+                       bt.leaf);
 }
 
 template void launch_block_inverses<double>(const double *, int64_t, int64_t, const double *, double *, const int *,
-                                            hipStream_t, bool);
+                                            hipStream_t, bool, const Batch &);
 template void launch_block_inverses<float>(const float *, int64_t, int64_t, const double *, double *, const int *,
-                                           hipStream_t, bool);
-template void launch_leaf_inverses<double>(const double *, int64_t, int64_t, double *, hipStream_t);
-template void launch_leaf_inverses<float>(const float *, int64_t, int64_t, double *, hipStream_t);
+                                           hipStream_t, bool, const Batch &);
+template void launch_leaf_inverses<double>(const double *, int64_t, int64_t, double *, hipStream_t, const Batch &);
+template void launch_leaf_inverses<float>(const float *, int64_t, int64_t, double *, hipStream_t, const Batch &);
 
 }  // namespace mi

@@ -66,8 +66,17 @@ template <bool BACKWARD, typename T, bool UC = false>
 __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ L, int64_t lda, int64_t n,
                                                           double *__restrict__ B, int64_t ldb, int64_t nrhs,
                                                           const double *__restrict__ Winv, double *Ypk, int *ticket,
-                                                          const int *info, int *status)
+                                                          const int *info, int *status, int64_t sL, int64_t sB,
+                                                          int64_t sWinv, int64_t sY, int64_t sT)
 {
+    // batched launches: system z = blockIdx.z, its pointers advanced by z times their strides
+    const int64_t z = blockIdx.z;
+    L += z * sL;
+    B += z * sB;
+    Winv += z * sWinv;
+    Ypk += z * sY;
+    ticket += z * sT;
+    if (info) info += z;
     // uniform: info and the status word are final before this kernel starts
     if (factor_failed(info) || device_failed(status)) return;
     // Ys[c + SC*k + 16 (k / 32)]: the published block being consumed, one padding row per
@@ -296,8 +305,11 @@ __global__ __launch_bounds__(512) void solve_sweep_kernel(const T *__restrict__ 
 }
 
 // Before a sweep: the ticket counter to 0 and the sweep's packed copies to kUnwritten.
-__global__ __launch_bounds__(256) void solve_reset_kernel(int *ticket, double *Ypk, int64_t count)
+__global__ __launch_bounds__(256) void solve_reset_kernel(int *ticket, double *Ypk, int64_t count, int64_t sT,
+                                                          int64_t sY)
 {
+    ticket += (int64_t)blockIdx.z * sT;
+    Ypk += (int64_t)blockIdx.z * sY;
     const int64_t i0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i0 == 0) *ticket = 0;
     const double u = __longlong_as_double((long long)kUnwritten);
@@ -308,34 +320,37 @@ __global__ __launch_bounds__(256) void solve_reset_kernel(int *ticket, double *Y
 template <typename T>
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, double *Ypk, int *tickets, const int *info, int *status, hipStream_t s,
-                             int sweeps, bool upper_copy)
+                             int sweeps, bool upper_copy, const Batch &bt)
 {
+    const unsigned nb = bt.count;
     const int nblk = (int)((n + SB - 1) / SB);
     const int ngrp = (int)((nrhs + SC - 1) / SC);
     const unsigned items = (unsigned)(nblk * ngrp);
     const int64_t pk = solve_pack_doubles(n, nrhs);
     const unsigned fill = (unsigned)((pk + 255) / 256 < 1024 ? (pk + 255) / 256 : 1024);
     if (sweeps & 1) {
-        hipLaunchKernelGGL(solve_reset_kernel, dim3(fill), dim3(256), 0, s, tickets, Ypk, pk);
-        hipLaunchKernelGGL((solve_sweep_kernel<false, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs, Winv,
-                           Ypk, tickets, info, status);
+        hipLaunchKernelGGL(solve_reset_kernel, dim3(fill, 1, nb), dim3(256), 0, s, tickets, Ypk, pk, bt.sync, bt.pack);
+        hipLaunchKernelGGL((solve_sweep_kernel<false, T>), dim3(items, 1, nb), dim3(512), 0, s, L, lda, n, B, ldb,
+                           nrhs, Winv, Ypk, tickets, info, status, bt.a, bt.b, bt.inv, bt.pack, bt.sync);
     }
     if (sweeps & 2) {
-        hipLaunchKernelGGL(solve_reset_kernel, dim3(fill), dim3(256), 0, s, tickets + 1, Ypk + pk, pk);
+        hipLaunchKernelGGL(solve_reset_kernel, dim3(fill, 1, nb), dim3(256), 0, s, tickets + 1, Ypk + pk, pk, bt.sync,
+                           bt.pack);
         if (upper_copy)
-            hipLaunchKernelGGL((solve_sweep_kernel<true, T, true>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb,
-                               nrhs, Winv, Ypk + pk, tickets + 1, info, status);
+            hipLaunchKernelGGL((solve_sweep_kernel<true, T, true>), dim3(items, 1, nb), dim3(512), 0, s, L, lda, n, B,
+                               ldb, nrhs, Winv, Ypk + pk, tickets + 1, info, status, bt.a, bt.b, bt.inv, bt.pack,
+                               bt.sync);
         else
-            hipLaunchKernelGGL((solve_sweep_kernel<true, T>), dim3(items), dim3(512), 0, s, L, lda, n, B, ldb, nrhs,
-                               Winv, Ypk + pk, tickets + 1, info, status);
+            hipLaunchKernelGGL((solve_sweep_kernel<true, T>), dim3(items, 1, nb), dim3(512), 0, s, L, lda, n, B, ldb,
+                               nrhs, Winv, Ypk + pk, tickets + 1, info, status, bt.a, bt.b, bt.inv, bt.pack, bt.sync);
     }
 }
 
 template void launch_solve_persistent<double>(const double *, int64_t, int64_t, double *, int64_t, int64_t,
                                               const double *, double *, int *, const int *, int *, hipStream_t, int,
-                                              bool);
+                                              bool, const Batch &);
 template void launch_solve_persistent<float>(const float *, int64_t, int64_t, double *, int64_t, int64_t,
                                              const double *, double *, int *, const int *, int *, hipStream_t, int,
-                                             bool);
+                                             bool, const Batch &);
 
 }  // namespace mi

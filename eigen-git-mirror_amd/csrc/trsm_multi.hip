@@ -34,8 +34,15 @@ namespace mi {
 template <bool UPPER>
 __global__ __launch_bounds__(256, 2) void trsm_xl_step_kernel(const double *__restrict__ Lt, int64_t ldlt,
                                                               int64_t n, double *__restrict__ B, int64_t ldb,
-                                                              int64_t nrow, const double *__restrict__ Winv, int i)
+                                                              int64_t nrow, const double *__restrict__ Winv, int i,
+                                                              int64_t sLt, int64_t sB, int64_t sWinv)
 {
+    {  // batched launches: system z = blockIdx.z
+        const int64_t z = blockIdx.z;
+        Lt += z * sLt;
+        B += z * sB;
+        Winv += z * sWinv;
+    }
     const int nblk = (int)((n + 127) / 128);
     const int nrt = (int)((nrow + 127) / 128);
     const int item = (int)blockIdx.x;
@@ -99,7 +106,8 @@ __global__ __launch_bounds__(256, 2) void trsm_xl_step_kernel(const double *__re
 
 // B (nrow x n, ld ldb) <- B L^{-1} with Lt = L^T; upper: only the block-upper tiles (nrow == n).
 void launch_trsm_xl_multi(const double *Lt, int64_t ldlt, int64_t n, double *B, int64_t ldb, int64_t nrow,
-                          const double *Winv, bool upper, hipStream_t s)
+                          const double *Winv, bool upper, hipStream_t s, unsigned count, int64_t sLt, int64_t sB,
+                          int64_t sWinv)
 {
     const int nblk = (int)((n + 127) / 128);
     const int nrt = (int)((nrow + 127) / 128);
@@ -108,12 +116,12 @@ void launch_trsm_xl_multi(const double *Lt, int64_t ldlt, int64_t n, double *B, 
         if (upper) {
             const int64_t prow = (128 * (int64_t)(i + 1)) < nrow ? 128 * (int64_t)(i + 1) : nrow;
             const unsigned items = (unsigned)((prow + 31) / 32 + (first ? 0 : i * (i + 1) / 2));
-            hipLaunchKernelGGL(trsm_xl_step_kernel<true>, dim3(items), dim3(256), 0, s, Lt, ldlt, n, B, ldb, nrow,
-                               Winv, i);
+            hipLaunchKernelGGL(trsm_xl_step_kernel<true>, dim3(items, 1, count), dim3(256), 0, s, Lt, ldlt, n, B, ldb,
+                               nrow, Winv, i, sLt, sB, sWinv);
         } else {
             const unsigned items = (unsigned)((nrow + 31) / 32 + (first ? 0 : (int64_t)i * nrt));
-            hipLaunchKernelGGL(trsm_xl_step_kernel<false>, dim3(items), dim3(256), 0, s, Lt, ldlt, n, B, ldb, nrow,
-                               Winv, i);
+            hipLaunchKernelGGL(trsm_xl_step_kernel<false>, dim3(items, 1, count), dim3(256), 0, s, Lt, ldlt, n, B, ldb,
+                               nrow, Winv, i, sLt, sB, sWinv);
         }
     }
 }

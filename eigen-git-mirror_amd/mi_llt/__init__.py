@@ -85,6 +85,12 @@ def lib() -> ctypes.CDLL:
                                ctypes.c_double, _VP, _I64]
         L.mi_dpocon.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.mi_dllt_rank_update.argtypes = [_VP, ctypes.c_char, _I64, _VP, _I64, _VP, _I64, ctypes.c_double, _VP]
+        L.mi_dpotrf_batched_async.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _I64, _VP]
+        L.mi_dpotrs_batched.argtypes = [_VP, ctypes.c_char, _I64, _I64, _I64, _VP, _I64, _I64, _VP, _I64, _I64]
+        L.mi_dpotrf_adjoint_batched.argtypes = [_VP, ctypes.c_char, _I64, _I64, _VP, _I64, _I64, _VP, _I64, _I64]
+        L.mi_dpotrs_adjoint_batched.argtypes = [_VP, ctypes.c_char, _I64, _I64, _I64, _VP, _I64, _I64, _VP, _I64,
+                                                _I64, _VP, _I64, _I64, _VP, _I64, _I64]
+        L.mi_dposv_noalloc.argtypes = [ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _VP]
         L.LAPACKE_spotrf.argtypes = [_INT, ctypes.c_char, _INT, _VP, _INT]
         L.LAPACKE_spotrs.argtypes = [_INT, ctypes.c_char, _INT, _INT, _VP, _INT, _VP, _INT]
         _PI = ctypes.POINTER(_INT)
@@ -109,7 +115,8 @@ def lib() -> ctypes.CDLL:
                      "mi_dposv_mixed", "mi_dposv_vbatched", "mi_sposv_vbatched", "mi_llt_set_vbatched_config",
                      "mi_llt_get_vbatched_config", "mi_dpotrf_adjoint", "mi_dpotrs_adjoint", "mi_spotrf",
                      "mi_spotrf_async", "mi_spotrs", "mi_dtrsm", "mi_dsyrk", "mi_dpocon", "mi_dllt_rank_update",
-                     "LAPACKE_spotrf", "LAPACKE_spotrs"):
+                     "LAPACKE_spotrf", "LAPACKE_spotrs", "mi_dpotrf_batched_async", "mi_dpotrs_batched",
+                     "mi_dpotrf_adjoint_batched", "mi_dpotrs_adjoint_batched", "mi_dposv_noalloc"):
             getattr(L, name).restype = _INT
         L.dpotrf_.restype = None
         L.dpotrs_.restype = None
@@ -124,6 +131,26 @@ def colmajor(t: torch.Tensor) -> torch.Tensor:
     out = torch.empty_strided(tuple(t.shape), (1, max(t.shape[0], 1)), dtype=t.dtype, device=t.device)
     out.copy_(t)
     return out
+
+
+def colmajor_batched(t: torch.Tensor) -> torch.Tensor:
+    """Copy of a (batch, rows, cols) tensor with every matrix column-major and packed:
+    strides (rows * cols, 1, rows), the layout of the batched entry points."""
+    if t.dim() == 2:
+        t = t.unsqueeze(-1)
+    b, r, c = (int(x) for x in t.shape)
+    out = torch.empty_strided((b, r, c), (r * c, 1, max(r, 1)), dtype=t.dtype, device=t.device)
+    out.copy_(t)
+    return out
+
+
+def _batched_ld(t: torch.Tensor):
+    """(ld, stride) of a batched column-major tensor (see colmajor_batched)."""
+    if t.dim() != 3 or (t.shape[1] > 1 and t.stride(1) != 1):
+        raise ValueError("expected a (batch, rows, cols) tensor of column-major matrices; use mi_llt.colmajor_batched()")
+    ld = max(int(t.stride(2)), int(t.shape[1]), 1)
+    stride = int(t.stride(0)) if t.shape[0] > 1 else ld * int(t.shape[2])
+    return ld, stride
 
 
 def _ld(t: torch.Tensor) -> int:
@@ -314,6 +341,25 @@ def rank_update_(l: torch.Tensor, v: torch.Tensor, sigma: float = 1.0, uplo: str
     return int(d_info.item())
 
 
+def potrf_batched_async_(a: torch.Tensor, d_info: torch.Tensor, handle: Handle | None = None):
+    """In-place Cholesky of every matrix of a (batch, n, n) float64 tensor (column-major per
+    matrix, lower triangle; ``mi_dpotrf_batched_async``); ``d_info``: int32 device tensor of
+    ``batch`` entries (LAPACK info per system)."""
+    h = handle.bind_stream() if handle else default_handle()
+    ld, st = _batched_ld(a)
+    h.check(lib().mi_dpotrf_batched_async(h.ptr, b"L", int(a.shape[1]), int(a.shape[0]), _VP(a.data_ptr()), ld, st,
+                                          _VP(d_info.data_ptr())), "mi_dpotrf_batched_async")
+
+
+def potrs_batched_(l: torch.Tensor, b: torch.Tensor, handle: Handle | None = None):
+    """In-place ``b[z] <- A[z]^{-1} b[z]`` with the factors in ``l`` (``mi_dpotrs_batched``)."""
+    h = handle.bind_stream() if handle else default_handle()
+    ldl, sl = _batched_ld(l)
+    ldb, sb = _batched_ld(b)
+    h.check(lib().mi_dpotrs_batched(h.ptr, b"L", int(l.shape[1]), int(b.shape[2]), int(l.shape[0]),
+                                    _VP(l.data_ptr()), ldl, sl, _VP(b.data_ptr()), ldb, sb), "mi_dpotrs_batched")
+
+
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16x3": 3, "bf16x6": 4}
 
 
@@ -350,6 +396,6 @@ def sy_l1norm(a: torch.Tensor, uplo: str = "L", handle: Handle | None = None) ->
 
 from .llt import LLT, ComputationInfo  # noqa: E402
 
-__all__ = ["lib", "colmajor", "Handle", "default_handle", "potrf_", "potrf_async_", "potrs_", "posv_async_",
+__all__ = ["lib", "colmajor", "colmajor_batched", "potrf_batched_async_", "potrs_batched_", "Handle", "default_handle", "potrf_", "potrf_async_", "potrs_", "posv_async_",
            "posv_mixed", "PREC", "sy_l1norm", "spotrf_", "spotrs_", "trsm_", "syrk_", "pocon", "rank_update_",
            "LLT", "ComputationInfo", "MiLLTError", "LIB_PATH", "E_DEVICE"]

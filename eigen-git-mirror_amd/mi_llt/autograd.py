@@ -7,6 +7,9 @@ symmetric input is returned as a full symmetric matrix.
 
 * :class:`Cholesky`      A -> L (lower, upper triangle zero), backward = mi_dpotrf_adjoint
 * :class:`CholeskySolve` (B, L) -> X = (L L^T)^{-1} B, backward = mi_dpotrs_adjoint
+* :class:`CholeskyBatched`, :class:`CholeskySolveBatched`, :func:`gp_log_marginal_batched`:
+  the same for a (batch, n, n) stack of systems of one order, through the batched entry
+  points (every kernel launch covers the whole batch)
 * :func:`gp_log_marginal` the Gaussian-process log marginal likelihood
   -1/2 y^T K^{-1} y - sum log L_ii - n/2 log(2 pi), whose gradient is 1/2 (a a^T - K^{-1}),
   a = K^{-1} y.
@@ -18,7 +21,8 @@ import math
 
 import torch
 
-from . import _VP, _ld, colmajor, default_handle, lib, potrf_, potrs_
+from . import (_VP, _batched_ld, _ld, colmajor, colmajor_batched, default_handle, lib, potrf_, potrf_batched_async_,
+               potrs_, potrs_batched_)
 
 
 def _cm(t: torch.Tensor) -> torch.Tensor:
@@ -88,3 +92,81 @@ def gp_log_marginal(k: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
     l = cholesky(k)
     alpha = cholesky_solve(y, l)
     return -0.5 * torch.dot(y, alpha) - torch.log(torch.diagonal(l)).sum() - 0.5 * n * math.log(2.0 * math.pi)
+
+
+class CholeskyBatched(torch.autograd.Function):
+    """A (batch, n, n) -> L (batch, n, n) lower; backward = mi_dpotrf_adjoint_batched."""
+
+    @staticmethod
+    def forward(ctx, a: torch.Tensor) -> torch.Tensor:
+        l = colmajor_batched(a.to(torch.float64))
+        info = torch.zeros(l.shape[0], dtype=torch.int32, device=l.device)
+        potrf_batched_async_(l, info)
+        bad = torch.nonzero(info).flatten()
+        if bad.numel():
+            z = int(bad[0])
+            raise torch.linalg.LinAlgError(f"Cholesky: system {z}: the leading minor of order {int(info[z])} is not "
+                                           "positive definite")
+        l = colmajor_batched(torch.tril(l))
+        ctx.save_for_backward(l)
+        return l
+
+    @staticmethod
+    def backward(ctx, lbar: torch.Tensor) -> torch.Tensor:
+        (l,) = ctx.saved_tensors
+        g = colmajor_batched(torch.tril(lbar.to(torch.float64)))
+        h = default_handle()
+        ldl, sl = _batched_ld(l)
+        ldg, sg = _batched_ld(g)
+        h.check(lib().mi_dpotrf_adjoint_batched(h.ptr, b"L", int(l.shape[1]), int(l.shape[0]), _VP(l.data_ptr()), ldl,
+                                                sl, _VP(g.data_ptr()), ldg, sg), "mi_dpotrf_adjoint_batched")
+        return g
+
+
+class CholeskySolveBatched(torch.autograd.Function):
+    """(B (batch, n, r), L) -> X = (L L^T)^{-1} B per system; backward = mi_dpotrs_adjoint_batched."""
+
+    @staticmethod
+    def forward(ctx, b: torch.Tensor, l: torch.Tensor) -> torch.Tensor:
+        vec = b.dim() == 2
+        x = colmajor_batched(b.to(torch.float64))
+        lc = colmajor_batched(l.to(torch.float64))
+        potrs_batched_(lc, x)
+        ctx.save_for_backward(lc, x)
+        ctx.vec = vec
+        return x.squeeze(-1) if vec else x
+
+    @staticmethod
+    def backward(ctx, xbar: torch.Tensor):
+        lc, x = ctx.saved_tensors
+        bbar = colmajor_batched(xbar.to(torch.float64).reshape(x.shape))
+        lbar = colmajor_batched(torch.zeros_like(lc))
+        h = default_handle()
+        ldl, sl = _batched_ld(lc)
+        ldx, sx = _batched_ld(x)
+        ldb, sb = _batched_ld(bbar)
+        ldlb, slb = _batched_ld(lbar)
+        h.check(lib().mi_dpotrs_adjoint_batched(h.ptr, b"L", int(lc.shape[1]), int(x.shape[2]), int(lc.shape[0]),
+                                                _VP(lc.data_ptr()), ldl, sl, _VP(x.data_ptr()), ldx, sx,
+                                                _VP(bbar.data_ptr()), ldb, sb, _VP(lbar.data_ptr()), ldlb, slb),
+                "mi_dpotrs_adjoint_batched")
+        bb = bbar.squeeze(-1) if ctx.vec else bbar
+        return bb, lbar
+
+
+def cholesky_batched(a: torch.Tensor) -> torch.Tensor:
+    return CholeskyBatched.apply(a)
+
+
+def cholesky_solve_batched(b: torch.Tensor, l: torch.Tensor) -> torch.Tensor:
+    return CholeskySolveBatched.apply(b, l)
+
+
+def gp_log_marginal_batched(k: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+    """log N(y_z | 0, K_z) for a (batch, n, n) stack K and (batch, n) targets y, through the
+    batched factorization and solve (differentiable in K): a (batch,) tensor."""
+    n = k.shape[-1]
+    l = cholesky_batched(k)
+    alpha = cholesky_solve_batched(y, l)
+    return (-0.5 * (y * alpha).sum(-1) - torch.log(torch.diagonal(l, dim1=-2, dim2=-1)).sum(-1)
+            - 0.5 * n * math.log(2.0 * math.pi))

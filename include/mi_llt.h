@@ -247,6 +247,31 @@ int mi_dpotrf_adjoint(mi_llt_handle h, char uplo, int64_t n, const double *dL, i
 int mi_dpotrs_adjoint(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const double *dL, int64_t ldl,
                       const double *dX, int64_t ldx, double *dXbar, int64_t ldxb, double *dLbar, int64_t ldlb);
 
+/* ---- batched, uniform order -------------------------------------------------------------
+ * Counterpart: a loop over independent systems of LLT::compute (LLT.h:430-460), LLT::solve
+ * (LLT.h:483-498) and the two reverse-mode entry points above.  `batch` (0..65535) systems of
+ * the same order n, system z's matrix at dA + z * strideA (likewise every array, strides in
+ * elements; systems must not overlap: stride >= ld * columns), uplo 'L' only
+ * (MI_LLT_E_UNSUPPORTED for 'U').  Every kernel launch covers all systems, so a batch of small
+ * systems (e.g. 256 of order 512) fills the GPU where a loop of single calls would not.
+ * Results are those of the single-system entry points, system by system.  Asynchronous.
+ * mi_dpotrf_batched_async: d_info is a device array of `batch` ints (LAPACK info per system).
+ * mi_dpotrs_batched: solve with each system's factor as left by mi_dpotrf_batched_async.
+ * mi_dpotrf_adjoint_batched / mi_dpotrs_adjoint_batched: mi_dpotrf_adjoint / mi_dpotrs_adjoint
+ *   per system.  Device workspace per system: 0.82 MiB per 128 rows (panel images and block
+ *   inverses) plus 3 n^2 doubles (factor adjoint) or 2 n nrhs doubles and the solve's packed
+ *   copies (solve adjoint). */
+int mi_dpotrf_batched_async(mi_llt_handle h, char uplo, int64_t n, int64_t batch, double *dA, int64_t lda,
+                            int64_t strideA, int *d_info);
+int mi_dpotrs_batched(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, int64_t batch, const double *dL,
+                      int64_t ldl, int64_t strideL, double *dB, int64_t ldb, int64_t strideB);
+int mi_dpotrf_adjoint_batched(mi_llt_handle h, char uplo, int64_t n, int64_t batch, const double *dL, int64_t ldl,
+                              int64_t strideL, double *dG, int64_t ldg, int64_t strideG);
+int mi_dpotrs_adjoint_batched(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, int64_t batch, const double *dL,
+                              int64_t ldl, int64_t strideL, const double *dX, int64_t ldx, int64_t strideX,
+                              double *dXbar, int64_t ldxb, int64_t strideXbar, double *dLbar, int64_t ldlb,
+                              int64_t strideLbar);
+
 #ifdef __cplusplus
 }
 #endif

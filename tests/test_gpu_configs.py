@@ -11,7 +11,8 @@
 4. 20,000 systems, n ~ U{8..256}, 1..8 right-hand sides, FP32: every system's factor and a
    sample of every size class's solves within FP32 bounds of the FP64 arithmetic (the FP64
    variant is test_vbatched_full_workload_backward_error in test_gpu_batched.py).
-5. The Gaussian-process gradient dL/dK for a batch of 256 kernels of n = 512: every gradient
+5. The Gaussian-process gradient dL/dK for a batch of 256 kernels of n = 512, computed in one
+   batched forward + backward (every kernel launch covers all 256 systems): every gradient
    against the closed form 1/2 (a a^T - K^{-1}) from torch's FP64 inverse (an independent
    library routine, used as a checker only), and a sample against the same gradient computed
    by Eigen's LLT on the CPU (oracle/_ref).  (The single n = 4096 case: test_gpu_adjoint.py.)
@@ -145,22 +146,23 @@ def test_config4_fp32_20000_systems(dev):
 
 
 def test_config5_gp_gradient_batch_256_n512(dev):
-    say("[gpu] config 5: 256 GP gradients, n=512")
+    say("[gpu] config 5: 256 GP gradients, n=512, one batched forward + backward")
     n, batch = 512, 256
+    ks, ys = zip(*[workloads.gp_kernel_torch(n, seed=100 + i, device=dev) for i in range(batch)])
+    k = torch.stack(ks).requires_grad_(True)
+    y = torch.stack(ys)
+    mad.gp_log_marginal_batched(k, y).sum().backward()
     worst = 0.0
     grads = []
     for i in range(batch):
-        k, y = workloads.gp_kernel_torch(n, seed=100 + i, device=dev)
-        k1 = k.clone().requires_grad_(True)
-        mad.gp_log_marginal(k1, y).backward()
-        kinv = torch.linalg.inv(sym(k))
-        alpha = kinv @ y
+        kinv = torch.linalg.inv(sym(ks[i]))
+        alpha = kinv @ ys[i]
         ref = 0.5 * (torch.outer(alpha, alpha) - kinv)
-        worst = max(worst, float((k1.grad - ref).abs().max() / ref.abs().max()))
+        worst = max(worst, float((k.grad[i] - ref).abs().max() / ref.abs().max()))
         if i in (0, 85, 170, 255):
-            grads.append((i, k.cpu().numpy(), y.cpu().numpy(), k1.grad.cpu().numpy()))
+            grads.append((i, ks[i].cpu().numpy(), ys[i].cpu().numpy(), k.grad[i].cpu().numpy()))
     assert worst <= 1e-9, worst
     if oracle.have_eigen():
-        for i, k, y, g in grads:  # the gradient computed by Eigen's LLT on the CPU
-            ge = oracle.eigen_gp_grad(k, y)
+        for i, kk, yy, g in grads:  # the gradient computed by Eigen's LLT on the CPU
+            ge = oracle.eigen_gp_grad(kk, yy)
             assert np.abs(g - ge).max() <= 1e-9 * np.abs(ge).max(), i
