@@ -293,8 +293,8 @@ def leg_adjoint(n=4096, nb=512, batch=256, reps=3):
                     "frac_fp64_peak": round((fl_f + fl_a) / tb / 1e9 / PEAK_FP64_TFLOPS, 4),
                     "adjoint_tflops": round(fl_a / tba / 1e9, 3),
                     "adjoint_frac_fp64_peak": round(fl_a / tba / 1e9 / PEAK_FP64_TFLOPS, 4),
-                    "path": "mi_dpotrf_batched_async / mi_dpotrs_batched forward, "
-                            "mi_dpotrs_adjoint_batched / mi_dpotrf_adjoint_batched backward"}
+                    "path": "mi_dposv_batched_async forward, "
+                            "mi_dposv_adjoint_batched backward"}
     return out
 
 

@@ -1343,3 +1343,101 @@ int mi_dpotrs_adjoint_batched(mi_llt_handle h, char uplo, int64_t n, int64_t nrh
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int mi_dposv_batched_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, int64_t batch, double *dA,
+                           int64_t lda, int64_t strideA, double *dB, int64_t ldb, int64_t strideB, int *d_info)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (batch < 0 || batch > 65535) return -5;
+    if (int e = check_batch_args(n, n, batch, lda, strideA, dA)) return e == 2 ? -6 : (e == 3 ? -7 : -8);
+    if (int e = check_batch_args(n, nrhs, batch, ldb, strideB, dB)) return e == 2 ? -9 : (e == 3 ? -10 : -11);
+    if (d_info == nullptr && batch > 0) return -12;
+    if (batch == 0) return 0;
+    hipStream_t s = h->stream;
+    hipLaunchKernelGGL(set_ints_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, s, d_info, (int)batch, 0);
+    if (n == 0) return check_launch(h, "set_info");
+    BatchLayout bl;
+    int rc = prepare_batch(h, n, nrhs, batch, nrhs > 0, 0, bl);
+    if (rc) return rc;
+    bl.bt.a = strideA;
+    bl.bt.b = strideB;
+    mi::enqueue_factorization<double>(
+        s, dA, lda, n, bl.leaf, d_info, batch_sync(h), shared_status(h), MI_PREC_FP64, nullptr,
+        [&](double fl, auto &&launch) { traced(h, s, MI_TRACE_POTRF, fl * (double)batch, launch); }, bl.bt);
+    if (nrhs > 0) {
+        // the factor's panel images carry the leaf inverses the block inverses start from
+        traced(h, s, MI_TRACE_TRTRI, (double)n * mi::kNB * mi::kNB / 3.0 * (double)batch,
+               [&] { mi::launch_block_inverses<double>(dA, lda, n, bl.leaf, bl.inv, d_info, s, true, bl.bt); });
+        traced(h, s, MI_TRACE_SOLVE, 2.0 * (double)n * (double)n * (double)nrhs * (double)batch, [&] {
+            mi::launch_solve_persistent<double>(dA, lda, n, dB, ldb, nrhs, bl.inv, bl.pack,
+                                                (int *)batch_sync(h) + mi::kSyncSolveTickets, d_info,
+                                                shared_status(h), s, 3, false, bl.bt);
+        });
+    }
+    return check_launch(h, "posv_batched");
+}
+
+int mi_dposv_adjoint_batched(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, int64_t batch, const double *dL,
+                             int64_t ldl, int64_t strideL, const double *dX, int64_t ldx, int64_t strideX,
+                             double *dXbar, int64_t ldxb, int64_t strideXbar, double *dLbar, int64_t ldlb,
+                             int64_t strideLbar)
+{
+    if (!valid(h)) return MI_LLT_E_HANDLE;
+    const int up = uplo_code(uplo);
+    if (up < 0) return -2;
+    if (up == 1) return MI_LLT_E_UNSUPPORTED;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (batch < 0 || batch > 65535) return -5;
+    if (int e = check_batch_args(n, n, batch, ldl, strideL, dL)) return e == 2 ? -6 : (e == 3 ? -7 : -8);
+    if (int e = check_batch_args(n, nrhs, batch, ldx, strideX, dX)) return e == 2 ? -9 : (e == 3 ? -10 : -11);
+    if (int e = check_batch_args(n, nrhs, batch, ldxb, strideXbar, dXbar)) return e == 2 ? -12 : (e == 3 ? -13 : -14);
+    if (int e = check_batch_args(n, n, batch, ldlb, strideLbar, dLbar)) return e == 2 ? -15 : (e == 3 ? -16 : -17);
+    if (n == 0 || batch == 0) return 0;
+    const size_t sq = (size_t)n * (size_t)n, vec = (size_t)n * (size_t)nrhs;
+    BatchLayout bl;
+    int rc = prepare_batch(h, n, nrhs, batch, true, (3 * sq > 2 * vec ? 3 * sq : 2 * vec), bl);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    const unsigned nb = (unsigned)batch;
+    const double nd = (double)n, bd = (double)batch;
+    bl.bt.a = strideL;
+    bl.bt.b = strideXbar;
+    // one set of diagonal-block inverses (with the solve's couplings) serves both adjoints
+    mi::launch_leaf_inverses<double>(dL, ldl, n, bl.leaf, s, bl.bt);
+    traced(h, s, MI_TRACE_TRTRI, nd * mi::kNB * mi::kNB / 3.0 * bd,
+           [&] { mi::launch_block_inverses<double>(dL, ldl, n, bl.leaf, bl.inv, nullptr, s, true, bl.bt); });
+    if (nrhs > 0) {  // the solve's adjoint: Bbar = A^{-1} Xbar, Lbar -= tril(Bbar (L^T X)^T + X (L^T Bbar)^T)
+        traced(h, s, MI_TRACE_SOLVE, 2.0 * nd * nd * (double)nrhs * bd, [&] {
+            mi::launch_solve_persistent<double>(dL, ldl, n, dXbar, ldxb, nrhs, bl.inv, bl.pack,
+                                                (int *)batch_sync(h) + mi::kSyncSolveTickets, nullptr,
+                                                shared_status(h), s, 3, false, bl.bt);
+        });
+        mi::launch_solve_adjoint_update(dL, ldl, n, nrhs, dX, ldx, dXbar, ldxb, dLbar, ldlb, bl.aux, bl.aux + vec, s,
+                                        nb, strideL, strideX, strideXbar, strideLbar, bl.bt.aux);
+    }
+    // the factor's adjoint of the accumulated Lbar, into dLbar (mi_dpotrf_adjoint_batched's steps)
+    const int64_t sa = bl.bt.aux;
+    double *W2 = bl.aux, *W3 = bl.aux + sq, *Wt = bl.aux + 2 * sq;
+    mi::launch_sym_tile(4, dL, ldl, n, Wt, n, s, nb, strideL, sa);
+    mi::launch_sym_tile(4, dLbar, ldlb, n, W3, n, s, nb, strideLbar, sa);
+    traced(h, s, MI_TRACE_GEMM, nd * nd * nd / 3.0 * bd,
+           [&] { mi::launch_trmm_tn_lower(Wt, n, W3, n, n, W3, n, W2, s, nb, sa, sa, sa, sa); });
+    mi::launch_sym_tile(0, W3, n, n, W2, n, s, nb, sa, sa);
+    traced(h, s, MI_TRACE_SOLVE, nd * nd * nd * bd,
+           [&] { mi::launch_trsm_xl_multi(Wt, n, n, W2, n, n, bl.inv, false, s, nb, sa, sa, bl.bt.inv); });
+    mi::launch_sym_tile(1, W2, n, n, W3, n, s, nb, sa, sa);
+    traced(h, s, MI_TRACE_SOLVE, nd * nd * nd / 2.0 * bd,
+           [&] { mi::launch_trsm_xl_multi(Wt, n, n, W3, n, n, bl.inv, true, s, nb, sa, sa, bl.bt.inv); });
+    mi::launch_sym_tile(3, W3, n, n, dLbar, ldlb, s, nb, sa, strideLbar);
+    return check_launch(h, "posv_adjoint_batched");
+}
+
+}  // extern "C"

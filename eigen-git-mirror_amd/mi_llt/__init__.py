@@ -91,6 +91,10 @@ def lib() -> ctypes.CDLL:
         L.mi_dpotrs_adjoint_batched.argtypes = [_VP, ctypes.c_char, _I64, _I64, _I64, _VP, _I64, _I64, _VP, _I64,
                                                 _I64, _VP, _I64, _I64, _VP, _I64, _I64]
         L.mi_dposv_noalloc.argtypes = [ctypes.c_char, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _VP]
+        L.mi_dposv_batched_async.argtypes = [_VP, ctypes.c_char, _I64, _I64, _I64, _VP, _I64, _I64, _VP, _I64, _I64,
+                                             _VP]
+        L.mi_dposv_adjoint_batched.argtypes = [_VP, ctypes.c_char, _I64, _I64, _I64, _VP, _I64, _I64, _VP, _I64,
+                                               _I64, _VP, _I64, _I64, _VP, _I64, _I64]
         L.LAPACKE_spotrf.argtypes = [_INT, ctypes.c_char, _INT, _VP, _INT]
         L.LAPACKE_spotrs.argtypes = [_INT, ctypes.c_char, _INT, _INT, _VP, _INT, _VP, _INT]
         _PI = ctypes.POINTER(_INT)
@@ -116,7 +120,8 @@ def lib() -> ctypes.CDLL:
                      "mi_llt_get_vbatched_config", "mi_dpotrf_adjoint", "mi_dpotrs_adjoint", "mi_spotrf",
                      "mi_spotrf_async", "mi_spotrs", "mi_dtrsm", "mi_dsyrk", "mi_dpocon", "mi_dllt_rank_update",
                      "LAPACKE_spotrf", "LAPACKE_spotrs", "mi_dpotrf_batched_async", "mi_dpotrs_batched",
-                     "mi_dpotrf_adjoint_batched", "mi_dpotrs_adjoint_batched", "mi_dposv_noalloc"):
+                     "mi_dpotrf_adjoint_batched", "mi_dpotrs_adjoint_batched", "mi_dposv_noalloc",
+                     "mi_dposv_batched_async", "mi_dposv_adjoint_batched"):
             getattr(L, name).restype = _INT
         L.dpotrf_.restype = None
         L.dpotrs_.restype = None
@@ -351,6 +356,17 @@ def potrf_batched_async_(a: torch.Tensor, d_info: torch.Tensor, handle: Handle |
                                           _VP(d_info.data_ptr())), "mi_dpotrf_batched_async")
 
 
+def posv_batched_async_(a: torch.Tensor, b: torch.Tensor, d_info: torch.Tensor, handle: Handle | None = None):
+    """Factor every matrix of ``a`` (batch, n, n) in place and solve ``b`` (batch, n, r) in
+    place (``mi_dposv_batched_async``); ``d_info``: int32 device tensor of ``batch`` entries."""
+    h = handle.bind_stream() if handle else default_handle()
+    lda, sa = _batched_ld(a)
+    ldb, sb = _batched_ld(b)
+    h.check(lib().mi_dposv_batched_async(h.ptr, b"L", int(a.shape[1]), int(b.shape[2]), int(a.shape[0]),
+                                         _VP(a.data_ptr()), lda, sa, _VP(b.data_ptr()), ldb, sb,
+                                         _VP(d_info.data_ptr())), "mi_dposv_batched_async")
+
+
 def potrs_batched_(l: torch.Tensor, b: torch.Tensor, handle: Handle | None = None):
     """In-place ``b[z] <- A[z]^{-1} b[z]`` with the factors in ``l`` (``mi_dpotrs_batched``)."""
     h = handle.bind_stream() if handle else default_handle()
@@ -396,6 +412,7 @@ def sy_l1norm(a: torch.Tensor, uplo: str = "L", handle: Handle | None = None) ->
 
 from .llt import LLT, ComputationInfo  # noqa: E402
 
-__all__ = ["lib", "colmajor", "colmajor_batched", "potrf_batched_async_", "potrs_batched_", "Handle", "default_handle", "potrf_", "potrf_async_", "potrs_", "posv_async_",
+__all__ = ["lib", "colmajor", "colmajor_batched", "potrf_batched_async_", "potrs_batched_", "posv_batched_async_",
+           "Handle", "default_handle", "potrf_", "potrf_async_", "potrs_", "posv_async_",
            "posv_mixed", "PREC", "sy_l1norm", "spotrf_", "spotrs_", "trsm_", "syrk_", "pocon", "rank_update_",
            "LLT", "ComputationInfo", "MiLLTError", "LIB_PATH", "E_DEVICE"]

@@ -7,9 +7,11 @@ symmetric input is returned as a full symmetric matrix.
 
 * :class:`Cholesky`      A -> L (lower, upper triangle zero), backward = mi_dpotrf_adjoint
 * :class:`CholeskySolve` (B, L) -> X = (L L^T)^{-1} B, backward = mi_dpotrs_adjoint
-* :class:`CholeskyBatched`, :class:`CholeskySolveBatched`, :func:`gp_log_marginal_batched`:
-  the same for a (batch, n, n) stack of systems of one order, through the batched entry
-  points (every kernel launch covers the whole batch)
+* :class:`CholeskyBatched`, :class:`CholeskySolveBatched`: the same for a (batch, n, n) stack
+  of systems of one order, through the batched entry points (every kernel launch covers the
+  whole batch); :class:`GPLogMarginalBatched` / :func:`gp_log_marginal_batched` the fused
+  log marginal likelihood of a batch (mi_dposv_batched_async forward, mi_dposv_adjoint_batched
+  backward)
 * :func:`gp_log_marginal` the Gaussian-process log marginal likelihood
   -1/2 y^T K^{-1} y - sum log L_ii - n/2 log(2 pi), whose gradient is 1/2 (a a^T - K^{-1}),
   a = K^{-1} y.
@@ -21,8 +23,8 @@ import math
 
 import torch
 
-from . import (_VP, _batched_ld, _ld, colmajor, colmajor_batched, default_handle, lib, potrf_, potrf_batched_async_,
-               potrs_, potrs_batched_)
+from . import (_VP, _batched_ld, _ld, colmajor, colmajor_batched, default_handle, lib, posv_batched_async_, potrf_,
+               potrf_batched_async_, potrs_, potrs_batched_)
 
 
 def _cm(t: torch.Tensor) -> torch.Tensor:
@@ -162,11 +164,50 @@ def cholesky_solve_batched(b: torch.Tensor, l: torch.Tensor) -> torch.Tensor:
     return CholeskySolveBatched.apply(b, l)
 
 
+class GPLogMarginalBatched(torch.autograd.Function):
+    """(K (batch, n, n), y (batch, n)) -> log N(y_z | 0, K_z) per system, fused: the forward is
+    one factor + solve per system (mi_dposv_batched_async: L and a = K^{-1} y), the backward one
+    call of mi_dposv_adjoint_batched with the gradients the log-likelihood sends into a and L
+    (abar = -g/2 y, Lbar = -g diag(1/L_ii)), which returns dL/dK = g/2 (a a^T - K^{-1})."""
+
+    @staticmethod
+    def forward(ctx, k: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+        n = int(k.shape[-1])
+        l = colmajor_batched(k.detach().to(torch.float64))  # the lower triangle becomes L
+        a = colmajor_batched(y.detach().to(torch.float64))
+        info = torch.zeros(l.shape[0], dtype=torch.int32, device=l.device)
+        posv_batched_async_(l, a, info)
+        bad = torch.nonzero(info).flatten()
+        if bad.numel():
+            z = int(bad[0])
+            raise torch.linalg.LinAlgError(f"Cholesky: system {z}: the leading minor of order {int(info[z])} is not "
+                                           "positive definite")
+        d = torch.diagonal(l, dim1=-2, dim2=-1)
+        ya = a.squeeze(-1)
+        ctx.save_for_backward(l, a, y.detach().to(torch.float64))
+        return -0.5 * (y.detach() * ya).sum(-1) - torch.log(d).sum(-1) - 0.5 * n * math.log(2.0 * math.pi)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        l, a, y = ctx.saved_tensors
+        b, n = int(l.shape[0]), int(l.shape[1])
+        g = g.to(torch.float64)
+        xbar = colmajor_batched((-0.5 * g)[:, None] * y)
+        lbar = torch.zeros_like(l)  # same (column-major) strides as l
+        torch.diagonal(lbar, dim1=-2, dim2=-1).copy_(-g[:, None] / torch.diagonal(l, dim1=-2, dim2=-1))
+        h = default_handle()
+        ldl, sl = _batched_ld(l)
+        ldx, sx = _batched_ld(a)
+        ldb, sb = _batched_ld(xbar)
+        ldlb, slb = _batched_ld(lbar)
+        h.check(lib().mi_dposv_adjoint_batched(h.ptr, b"L", n, 1, b, _VP(l.data_ptr()), ldl, sl, _VP(a.data_ptr()), ldx,
+                                               sx, _VP(xbar.data_ptr()), ldb, sb, _VP(lbar.data_ptr()), ldlb, slb),
+                "mi_dposv_adjoint_batched")
+        return lbar, None
+
+
 def gp_log_marginal_batched(k: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
-    """log N(y_z | 0, K_z) for a (batch, n, n) stack K and (batch, n) targets y, through the
-    batched factorization and solve (differentiable in K): a (batch,) tensor."""
-    n = k.shape[-1]
-    l = cholesky_batched(k)
-    alpha = cholesky_solve_batched(y, l)
-    return (-0.5 * (y * alpha).sum(-1) - torch.log(torch.diagonal(l, dim1=-2, dim2=-1)).sum(-1)
-            - 0.5 * n * math.log(2.0 * math.pi))
+    """log N(y_z | 0, K_z) for a (batch, n, n) stack K and (batch, n) targets y: a (batch,)
+    tensor, differentiable in K (one batched factor + solve forward, one batched adjoint
+    backward)."""
+    return GPLogMarginalBatched.apply(k, y)

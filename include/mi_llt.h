@@ -271,6 +271,21 @@ int mi_dpotrs_adjoint_batched(mi_llt_handle h, char uplo, int64_t n, int64_t nrh
                               int64_t ldl, int64_t strideL, const double *dX, int64_t ldx, int64_t strideX,
                               double *dXbar, int64_t ldxb, int64_t strideXbar, double *dLbar, int64_t ldlb,
                               int64_t strideLbar);
+/* mi_dposv_batched_async: factor and solve per system (LLT(A_z).solve(B_z), LLT.h:104-110 +
+ *   483-498), the solve reusing the factor's panel images; B_z unchanged when system z's factor
+ *   fails (d_info[z] = k+1).
+ * mi_dposv_adjoint_batched: the reverse mode of that call.  With L as left in dA and X the
+ *   solution: on entry dXbar holds Xbar and dLbar (lower triangle) any gradient already
+ *   flowing into L (zero if none); on return dXbar holds Bbar = A^{-1} Xbar and dLbar holds
+ *   Abar (full symmetric): the factor adjoint of Lbar - tril(Bbar (L^T X)^T + X (L^T Bbar)^T),
+ *   i.e. mi_dpotrs_adjoint followed by mi_dpotrf_adjoint, sharing one set of diagonal-block
+ *   inverses.  Workspace per system as above with max(3 n^2, 2 n nrhs) doubles of scratch. */
+int mi_dposv_batched_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, int64_t batch, double *dA,
+                           int64_t lda, int64_t strideA, double *dB, int64_t ldb, int64_t strideB, int *d_info);
+int mi_dposv_adjoint_batched(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, int64_t batch, const double *dL,
+                             int64_t ldl, int64_t strideL, const double *dX, int64_t ldx, int64_t strideX,
+                             double *dXbar, int64_t ldxb, int64_t strideXbar, double *dLbar, int64_t ldlb,
+                             int64_t strideLbar);
 
 #ifdef __cplusplus
 }

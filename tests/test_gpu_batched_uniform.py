@@ -113,6 +113,44 @@ def test_batched_adjoints_match_single(dev):
     assert h.status() == 0
 
 
+@pytest.mark.parametrize("n,batch,nrhs", [(96, 3, 1), (512, 5, 1), (700, 2, 4)])
+def test_batched_posv_and_its_adjoint_match_single(dev, n, batch, nrhs):
+    say(f"[gpu] batched posv + posv adjoint n={n} batch={batch} nrhs={nrhs}")
+    a, mats = stack(n, batch, seed=40 + n)
+    b = mi_llt.colmajor_batched(torch.from_numpy(np.stack([workloads.rhs(n, nrhs, seed=70 + z)
+                                                           for z in range(batch)])).to(dev))
+    info = torch.full((batch,), 5, dtype=torch.int32, device=dev)
+    x = b.clone()
+    mi_llt.posv_batched_async_(a, x, info)
+    g = torch.Generator(device="cpu").manual_seed(9)
+    xbar = mi_llt.colmajor_batched(torch.randn(batch, n, nrhs, generator=g, dtype=torch.float64).to(dev))
+    lbar0 = mi_llt.colmajor_batched(torch.tril(torch.randn(batch, n, n, generator=g, dtype=torch.float64)).to(dev))
+    xb, lb = xbar.clone(), lbar0.clone()
+    h = mi_llt.default_handle()
+    L = mi_llt.lib()
+    vp = ctypes.c_void_p
+    assert L.mi_dposv_adjoint_batched(h.ptr, b"L", n, nrhs, batch, vp(a.data_ptr()), n, n * n, vp(x.data_ptr()), n,
+                                      n * nrhs, vp(xb.data_ptr()), n, n * nrhs, vp(lb.data_ptr()), n, n * n) == 0
+    torch.cuda.synchronize()
+    assert info.tolist() == [0] * batch and h.status() == 0
+    for z in range(batch):
+        a1 = mi_llt.colmajor(torch.from_numpy(mats[z]).to(dev))
+        x1 = mi_llt.colmajor(b[z].clone())
+        i1 = torch.zeros(1, dtype=torch.int32, device=dev)
+        mi_llt.posv_async_(a1, x1, i1)
+        torch.cuda.synchronize()
+        assert torch.equal(torch.tril(a[z]), torch.tril(a1)) and torch.equal(x[z], x1), f"posv of system {z}"
+        # single-system reverse mode: the solve's adjoint, then the factor's on the accumulated Lbar
+        xb1 = mi_llt.colmajor(xbar[z].clone())
+        lb1 = mi_llt.colmajor(lbar0[z].clone())
+        assert L.mi_dpotrs_adjoint(h.ptr, b"L", n, nrhs, vp(a1.data_ptr()), n, vp(x1.data_ptr()), n,
+                                   vp(xb1.data_ptr()), n, vp(lb1.data_ptr()), n) == 0
+        assert L.mi_dpotrf_adjoint(h.ptr, b"L", n, vp(a1.data_ptr()), n, vp(lb1.data_ptr()), n) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(xb[z], xb1), f"Bbar of system {z}"
+        assert torch.equal(lb[z], lb1), f"Abar of system {z}"
+
+
 def test_batched_gp_gradient_closed_form_and_single(dev):
     n, batch = 200, 6
     say(f"[gpu] batched GP gradient n={n} batch={batch}")
