@@ -52,7 +52,7 @@ struct mi_llt_context {
     // size, so a captured graph keeps valid pointers) and the schedule parameters
     void *vb_ws = nullptr;
     size_t vb_bytes = 0;
-    int vb_wg_per_cu = 2, vb_group = 4, vb_lpt = 1;
+    int vb_wg_per_cu = 2, vb_group = 0, vb_lpt = 1;  // group 0: by precision (FP64 2, FP32 4)
     int cus = 0;
     double t_flops[MI_TRACE_KINDS] = {0};
     int64_t t_calls[MI_TRACE_KINDS] = {0};
@@ -340,7 +340,7 @@ int vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n, T *const
     const double fl_est = 0.0;
     traced(h, h->stream, MI_TRACE_BATCH, fl_est, [&] {
         mi::launch_vbatched<T>(batch, d_n, d_A, d_lda, d_nrhs, d_B, d_ldb, d_info, order, counter, scratch, (int)grid,
-                               h->vb_group, h->vb_lpt, h->stream);
+                               h->vb_group ? h->vb_group : (sizeof(T) == 8 ? 2 : 4), h->vb_lpt, h->stream);
     });
     return check_launch(h, "vbatched");
 }
@@ -1049,7 +1049,7 @@ int mi_llt_set_vbatched_config(mi_llt_handle h, int wg_per_cu, int group, int lo
 {
     if (!valid(h)) return MI_LLT_E_HANDLE;
     if (wg_per_cu < 1 || wg_per_cu > 64) return -2;
-    if (group != 2 && group != 4) return -3;
+    if (group != 0 && group != 2 && group != 4) return -3;
     h->vb_wg_per_cu = wg_per_cu;
     h->vb_group = group;
     h->vb_lpt = longest_first ? 1 : 0;

@@ -186,11 +186,15 @@ def gram_batch(count: int, nmin: int = 8, nmax: int = 256, rmin: int = 1, rmax: 
         a = (torch.matmul(g, g.transpose(1, 2).contiguous()) / n).to(dtype)
         a.diagonal(dim1=1, dim2=2).add_(shift)
         bv = torch.randn(len(idx), rmax, n, dtype=torch.float64, device=vb.device, generator=gen).to(dtype)
-        for i, k in enumerate(idx):
-            o = int(vb.a_off[k])
-            vb.a_flat[o:o + n * n] = a[i].reshape(-1)  # symmetric: row- and column-major agree
-            r = int(rs[k])
-            if r:
-                o = int(vb.b_off[k])
-                vb.b_flat[o:o + n * r] = bv[i, :r].reshape(-1)
+        # one scatter per size class (A_i symmetric: row- and column-major agree; B_i column-major
+        # n x r_i, the first r_i rows of bv[i])
+        ia = (vb.a_off[idx][:, None] + np.arange(n * n)[None, :]).reshape(-1)
+        vb.a_flat[torch.from_numpy(ia).to(vb.device)] = a.reshape(-1)
+        r_of = rs[idx]
+        sel = np.arange(rmax)[None, :] < r_of[:, None]                      # (systems, rmax)
+        ib = vb.b_off[idx][:, None, None] + (np.arange(rmax)[None, :, None] * n + np.arange(n)[None, None, :])
+        ib = ib[sel]                                                          # (sum r_i, n)
+        if ib.size:
+            vals = bv[torch.from_numpy(sel).to(vb.device)]                    # same order
+            vb.b_flat[torch.from_numpy(ib.reshape(-1)).to(vb.device)] = vals.reshape(-1)
     return vb

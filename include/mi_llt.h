@@ -226,7 +226,9 @@ int mi_sposv_vbatched(mi_llt_handle h, char uplo, int64_t batch, const int *d_n,
                       const int *d_lda, const int *d_nrhs, float *const *d_B, const int *d_ldb, int *d_info);
 /* Schedule of the batched launch (no reference counterpart; set by the autotuner in
  * mi_llt.batched): workgroups of 4 worker waves per CU (1..64), tile rows per register block
- * of the panel update (2 or 4), and longest-first ordering (1) or the given order (0). */
+ * of the panel update (2 or 4; 0, the default: 2 for FP64 and 4 for FP32, the largest each
+ * precision holds in registers without spilling), and longest-first ordering (1) or the given
+ * order (0). */
 int mi_llt_set_vbatched_config(mi_llt_handle h, int wg_per_cu, int group, int longest_first);
 int mi_llt_get_vbatched_config(mi_llt_handle h, int *wg_per_cu, int *group, int *longest_first);
 
