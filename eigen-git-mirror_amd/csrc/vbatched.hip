@@ -39,6 +39,24 @@ __device__ __forceinline__ float vb_readlane(float x, int lane)
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
 }
 
+// Global-memory views of a system's pointers.  A pointer rebuilt from readfirstlane'd integers
+// is a generic one, and generic (flat) loads count in both the vector-memory and the LDS
+// counters, which the compiler can only wait on as a whole: with flat loads it waited for the
+// prefetched k-step together with the current one.  Global loads are counted in order.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p)
+{
+    return (gptr<T>)p;
+}
+// Load at a byte offset from a wave-uniform base (scalar base + 32-bit vector offset form).
+template <typename T>
+__device__ __forceinline__ T load_at(gptr<const T> base, uint32_t off)
+{
+    return *(gptr<const T>)((__attribute__((address_space(1))) const char *)base + off);
+}
+
 template <typename T>
 struct VbArith;
 // FP64: v_mfma_f64_16x16x4f64; k-step s, lane group g supplies k = 4s + g
@@ -87,18 +105,16 @@ __device__ __forceinline__ void vb_fence()
 // j+1 go out before the MFMAs of step j (two register buffers used alternately by name, so
 // no buffer is indexed at run time).
 template <typename T, int G>
-__device__ int vb_system(int n, T *__restrict__ A, int64_t lda, int nrhs, T *__restrict__ B, int64_t ldb, T *Ls,
-                         T *Xs, T *__restrict__ Xg)
+__device__ int vb_system(int n, T *A_, int64_t lda, int nrhs, T *B_, int64_t ldb, T *Ls, T *Xs, T *Xg_)
 {
     using M = VbArith<T>;
     using acc_t = typename M::acc_t;
+    const gptr<T> A = to_global(A_), B = to_global(B_), Xg = to_global(Xg_);
     const int l = lane_id(), li = l & 15, g = l >> 4;
     const int nt = (n + 15) >> 4;
     const T zero = (T)0, one = (T)1;
     const int kx0 = M::kx(0, g);  // kx(s, g) = kx0 + M::step(s)
-    auto at = [](const T *base, uint32_t off) -> T {
-        return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
-    };
+    auto at = [](gptr<const T> base, uint32_t off) -> T { return load_at<T>(base, off); };
     auto lane_off = [&](int row, int64_t ld) -> uint32_t { return (uint32_t)((row + (int64_t)kx0 * ld) * (int64_t)sizeof(T)); };
 
     for (int k = 0; k < nt; ++k) {
@@ -196,7 +212,7 @@ __device__ int vb_system(int n, T *__restrict__ A, int64_t lda, int nrhs, T *__r
     do {                                                                                           \
         _Pragma("unroll") for (int s = 0; s < 4; ++s)                                              \
         {                                                                                          \
-            const T *base_ = A + (int64_t)(16 * (j_) + M::step(s)) * lda;                          \
+            const gptr<const T> base_ = A + (int64_t)(16 * (j_) + M::step(s)) * lda;               \
             q_[s] = at(base_, offq);                                                               \
             _Pragma("unroll") for (int u = 0; u < G; ++u) x_[u][s] = at(base_, offp[u]);            \
         }                                                                                          \
@@ -250,7 +266,7 @@ __device__ int vb_system(int n, T *__restrict__ A, int64_t lda, int nrhs, T *__r
         const int col = c0 + li;
         const bool cin = col < nrhs;
         const int64_t bcol = (int64_t)(cin ? col : c0) * ldb;
-        const T *Bc = B + bcol;
+        const gptr<const T> Bc = B + bcol;
         // forward: Y_k^T = (B_k^T - sum_{j<k} Y_j^T L_kj^T) X_kk^T
         for (int k = 0; k < nt; ++k) {
             const int k0 = 16 * k;
