@@ -263,7 +263,7 @@ int prepare_mixed(mi_llt_handle h, int64_t n, int64_t nrhs, bool fp64, MixedLayo
     const size_t part = round_up((size_t)mi::residual_chunks(n, nrhs) * (size_t)n * (size_t)nrhs * sizeof(double));
     const size_t tpart = round_up((size_t)mi::sym_tiles64(n) * sizeof(double));
     const size_t scal = round_up((size_t)(2 + 2 * nrhs) * sizeof(double));
-    const size_t split = fp64 ? 0 : round_up((size_t)(4 * mi::split_slot_elems(n, 3)) * sizeof(__bf16));
+    const size_t split = fp64 ? 0 : round_up((size_t)(mi::kSplitRing * mi::split_slot_elems(n, 3)) * sizeof(__bf16));
     int rc = reserve(h, wbytes + leaf + inv + 2 * vec + part + tpart + scal + split);
     if (rc) return rc;
     char *p = (char *)h->ws;

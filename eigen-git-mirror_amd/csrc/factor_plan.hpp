@@ -6,7 +6,9 @@
 //   launch -1 : panel 0 (diagonal factor + panel solve of block column 0)
 //   launch k  : panel k+1 (its update by panel k, diagonal factor, panel solve)
 //               + trailing update by panels k-1 and k (depth 256) of every other block
-//               column from k+2 on (factor_step.hip explains the alternation).
+//               column from k+2 on, or (FP32 storage) by panels k-1, k of column k+2 and
+//               panels k-3 .. k of every fourth column from k+4 (factor_step.hip explains
+//               both schedules).
 // The two halves of a launch touch disjoint data; everything they read was finished by an
 // earlier launch, so stream order is the only synchronisation the host needs.
 #pragma once
@@ -18,6 +20,16 @@ namespace mi {
 // the stream: a kernel, not a memset, so that a captured factorization replays it
 // (factor_step.hip).
 void launch_factor_reset(uint32_t *sync, hipStream_t s, const Batch &bt = Batch());
+
+// The trailing update of launch k in the four-panel schedule of the FP32-storage
+// factorizations (factor_step.hip): npair tiles of block column c0 = k+2 at depth K from
+// column slab, then nquad tiles of the block columns cq0 = k+4, k+8, ... at depth KQ from
+// column slabq.  k = -1: none.
+struct QuadPlan {
+    int c0 = 0, cq0 = 0, npair = 0, nquad = 0, K = 128, KQ = 128;
+    int64_t slab = 0, slabq = 0;
+};
+QuadPlan quad_plan(int64_t n, int64_t k);
 
 // Algorithmic flop counts (LAPACK convention): POTRF of b: b^3/3 + b^2/2 + b/6; TRSM m x b
 // panel: m b^2; panel update by the previous panel: 2 (n - k0) b NB minus its upper part.
@@ -48,12 +60,20 @@ void enqueue_factorization(hipStream_t s, T *A, int64_t lda, int64_t n, double *
     for (int64_t k = -1; k < nblk; ++k) {
         const int64_t kp = (k + 1 < nblk) ? (k + 1) * NB : -1;  // panel of this launch
         // trailing update of launch k: panels k-1, k (k >= 1; panel 0 alone for k = 0) to the
-        // block columns c = k+2, k+4, ... (all rows from c down)
-        const double K = k >= 1 ? 2.0 * NB : (double)NB;
+        // block columns c = k+2, k+4, ... (all rows from c down); with FP32 storage panels
+        // k-1, k to column k+2 and panels k-3 .. k to the columns k+4, k+8, ...
         double fl_upd = 0.0;
-        for (int64_t c = k + 2; k >= 0 && c < nblk; c += 2) {
+        auto column = [&](int64_t c, double K) {
             const double m = (double)(n - c * NB), w = m < NB ? m : (double)NB;
             fl_upd += 2.0 * K * (m * w - w * (w - 1) / 2);
+        };
+        if constexpr (sizeof(T) == 8) {
+            const double K = k >= 1 ? 2.0 * NB : (double)NB;
+            for (int64_t c = k + 2; k >= 0 && c < nblk; c += 2) column(c, K);
+        } else if (k >= 0) {
+            const QuadPlan q = quad_plan(n, k);
+            if (q.npair > 0) column(q.c0, (double)q.K);
+            for (int64_t c = q.cq0; c < nblk; c += 4) column(c, (double)q.KQ);
         }
         if (kp < 0 && fl_upd == 0.0) break;
         const double fl = (kp >= 0 ? panel_flops(n, kp, NB) : 0.0) + fl_upd;

@@ -9,6 +9,7 @@
 #pragma once
 #include "common.hpp"
 #include "mfma_tile.hpp"
+#include "kernels.hpp"
 
 namespace mi {
 
@@ -43,9 +44,9 @@ __device__ __forceinline__ void syrk_tile(T *__restrict__ C, int64_t ldc, const 
 
 // The split forms (FP32 storage) read from LDS: PARTS = 2 (hi, lo: ~16 significant bits; three
 // products hi.hi, hi.lo, lo.hi) or 3 (hi, mid, lo: all 24 bits; six products hi.hi, hi.mid,
-// mid.hi, hi.lo, lo.hi, mid.mid -- the dropped terms are below 2^-24 of the product).  Parts of
-// the slots S0 (depth columns 0..127) and S1 (128..255), part t at + t * part_off; C's row 0
-// is row r0 of the slots.  Each k-step's operands (128 P rows and 128 Q rows x PARTS, 16 KiB
+// mid.hi, hi.lo, lo.hi, mid.mid -- the dropped terms are below 2^-24 of the product).  Depth
+// columns 128 q .. 128 q + 127 are those of the panel in ring slot (sp + q) % kSplitRing (at
+// ring + slot * slot_elems), part t at + t * part_off; C's row 0 is row r0 of the slots.  Each k-step's operands (128 P rows and 128 Q rows x PARTS, 16 KiB
 // per part) are staged once per workgroup in LDS (`lds` >= 32 KiB * PARTS; two waves read
 // every operand row), the next k-step's loads held in registers meanwhile (a raw 16-byte
 // vector type: HIP's uint4 struct would be kept in scratch memory), and the waves read their
@@ -53,8 +54,8 @@ __device__ __forceinline__ void syrk_tile(T *__restrict__ C, int64_t ldc, const 
 // four waves take part in the staging, so the strictly upper wave tile of a diagonal tile is
 // computed too (and not stored).
 template <int K, int PARTS>
-__device__ __forceinline__ void syrk_tile_split_lds(float *__restrict__ C, int64_t ldc, const __bf16 *S0,
-                                                    const __bf16 *S1, int64_t part_off, int r0, int m, int ncol,
+__device__ __forceinline__ void syrk_tile_split_lds(float *__restrict__ C, int64_t ldc, const __bf16 *ring, int64_t sp,
+                                                    int64_t slot_elems, int64_t part_off, int r0, int m, int ncol,
                                                     int ti, int tj, char *lds)
 {
     constexpr int S = K / 32, NC = 4 * PARTS;  // chunks per thread per k-step
@@ -70,7 +71,7 @@ __device__ __forceinline__ void syrk_tile_split_lds(float *__restrict__ C, int64
     // chunk i of a thread: operand i / (2 PARTS), group half (i / PARTS) % 2, part i % PARTS
 #define MI_SL_GLOAD(x, s_)                                                                                   \
     do {                                                                                                     \
-        const char *b_ = (const char *)((s_) < 4 ? S0 : S1) + 1024 * ((s_)&3);                              \
+        const char *b_ = (const char *)(ring + ((sp + (s_) / 4) & (kSplitRing - 1)) * slot_elems) + 1024 * ((s_)&3); \
         _Pragma("unroll") for (int i_ = 0; i_ < NC; ++i_) x[i_] =                                            \
             *(const u32x4 *)(b_ + ((i_ / (2 * PARTS)) ? gq : gp) + 16384u * ((i_ / PARTS) & 1) + pbytes * (i_ % PARTS)); \
     } while (0)

@@ -42,8 +42,12 @@ def test_mixed_reaches_fp64_backward_error(dev, prec, n, kappa):
         # kappa * u_fp32 <= 0.06: classical refinement converges without the FP64 fallback
         assert iters >= 1
     if prec == "bf16x6":
-        # the 3-part split carries all 24 bits of the FP32 working copy: FP32's convergence
-        assert 1 <= iters <= 3, iters
+        # the 3-part split carries all 24 bits of the FP32 working copy, and its products are
+        # accumulated in FP32 over slabs of up to 512 columns: it converges like the FP32
+        # factor of the same matrix, within one more correction
+        _, iters32, _, info32 = mi_llt.posv_mixed(a, b, prec="fp32", tol=TOL, maxit=30)
+        assert info32 == 0 and iters32 >= 1
+        assert 1 <= iters <= iters32 + 1, (iters, iters32)
     if prec == "bf16x3" and kappa <= 1e4:
         # split BF16 (~16 significant bits): a few corrections, no fallback -- a factor from
         # wrongly split operands would not converge this fast
