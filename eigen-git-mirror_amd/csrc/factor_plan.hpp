@@ -12,6 +12,7 @@
 // The two halves of a launch touch disjoint data; everything they read was finished by an
 // earlier launch, so stream order is the only synchronisation the host needs.
 #pragma once
+#include "common.hpp"
 #include "kernels.hpp"
 
 namespace mi {
@@ -25,11 +26,18 @@ void launch_factor_reset(uint32_t *sync, hipStream_t s, const Batch &bt = Batch(
 // factorizations (factor_step.hip): npair tiles of block column c0 = k+2 at depth K from
 // column slab, then nquad tiles of the block columns cq0 = k+4, k+8, ... at depth KQ from
 // column slabq.  k = -1: none.
+// oct (the 2-part split-BF16 arithmetic): block column k+4 alone takes the four panels, and
+// the block columns co0 = k+8, k+16, ... take the eight panels k-7 .. k (depth Ko from column
+// slabo, nocts tiles).
 struct QuadPlan {
     int c0 = 0, cq0 = 0, npair = 0, nquad = 0, K = 128, KQ = 128;
     int64_t slab = 0, slabq = 0;
+    bool oct = false;
+    int co0 = 0, nocts = 0, Ko = 128;
+    int64_t slabo = 0;
 };
-QuadPlan quad_plan(int64_t n, int64_t k);
+QuadPlan quad_plan(int64_t n, int64_t k, bool oct);
+inline bool oct_schedule(int prec) { return prec == kPrecBF16x3; }
 
 // Algorithmic flop counts (LAPACK convention): POTRF of b: b^3/3 + b^2/2 + b/6; TRSM m x b
 // panel: m b^2; panel update by the previous panel: 2 (n - k0) b NB minus its upper part.
@@ -71,9 +79,10 @@ void enqueue_factorization(hipStream_t s, T *A, int64_t lda, int64_t n, double *
             const double K = k >= 1 ? 2.0 * NB : (double)NB;
             for (int64_t c = k + 2; k >= 0 && c < nblk; c += 2) column(c, K);
         } else if (k >= 0) {
-            const QuadPlan q = quad_plan(n, k);
+            const QuadPlan q = quad_plan(n, k, oct_schedule(prec));
             if (q.npair > 0) column(q.c0, (double)q.K);
-            for (int64_t c = q.cq0; c < nblk; c += 4) column(c, (double)q.KQ);
+            for (int64_t c = q.cq0; c < nblk; c += q.oct ? nblk : 4) column(c, (double)q.KQ);
+            for (int64_t c = q.co0; q.oct && c < nblk; c += 8) column(c, (double)q.Ko);
         }
         if (kp < 0 && fl_upd == 0.0) break;
         const double fl = (kp >= 0 ? panel_flops(n, kp, NB) : 0.0) + fl_upd;

@@ -55,7 +55,9 @@ __device__ __forceinline__ void strided_tile(int u, int c0, int step, int nblk, 
 //     tiles (0: none); slab: first column of the K-wide operand (the last one or two panels).
 // KQ > 0 (the four-panel schedule): c0 is the one block column of the pair update (depth K,
 //     slab), and the nsyrk tiles continue over the block columns cq0, cq0 + 4, ... with
-//     depth KQ from column slabq.
+//     depth KQ from column slabq; with the 2-part split arithmetic (the eight-panel schedule)
+//     block column cq0 alone takes depth KQ, and the tiles continue over the block columns
+//     co0, co0 + 8, ... with nko k-steps of 32 from column slabo.
 // S: storage type (double, or float for the mixed-precision factorization), PREC: the
 // trailing-update arithmetic (kPrecF64 with double; FP32/BF16/split BF16 MFMA with float).
 // Wsplit (split BF16 arithmetic only, else null): a ring of 4 panel slots of slot_elems BF16
@@ -70,7 +72,7 @@ __device__ __forceinline__ void strided_tile(int u, int c0, int step, int nblk, 
 template <typename S, int PREC, int K, int KQ>
 __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S *__restrict__ A, int64_t lda, int64_t n, int64_t kp,
                                                           int Gp, int c0, int nsyrk, int64_t slab, int cq0,
-                                                          int64_t slabq,
+                                                          int64_t slabq, int co0, int64_t slabo, int nko,
                                                           double *__restrict__ Wleaf, int *info, uint32_t *tickets,
                                                           uint32_t ticket_base, uint32_t *flag, uint32_t epoch,
                                                           int *status, __bf16 *Wsplit, int64_t slot_elems,
@@ -114,18 +116,27 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
     }
     const int u = xcd_remap(Gp > 0 ? t - 1 : t, nsyrk);
     const int nblk = (int)((n + kNB - 1) / kNB);
+    constexpr bool OCT = KQ > 0 && PREC == kPrecBF16x3;
     int i, c;
-    bool quad = false;
+    bool quad = false, oct = false;
     if constexpr (KQ == 0) {
         strided_tile(u, c0, 2, nblk, i, c);
     } else {
         const int npair = c0 < nblk ? nblk - c0 : 0;
+        const int nq = cq0 < nblk ? nblk - cq0 : 0;
         if (u < npair) {
             c = c0;
             i = c0 + u;
-        } else {
+        } else if (!OCT) {
             strided_tile(u - npair, cq0, 4, nblk, i, c);
             quad = true;
+        } else if (u - npair < nq) {
+            c = cq0;
+            i = cq0 + (u - npair);
+            quad = true;
+        } else {
+            strided_tile(u - npair - nq, co0, 8, nblk, i, c);
+            oct = true;
         }
     }
     const int64_t r = (int64_t)c * kNB;
@@ -135,14 +146,15 @@ __global__ __launch_bounds__(256, MI_FACTOR_WG_PER_CU) void factor_step_kernel(S
         if (Wsplit) {
             constexpr int PARTS = PREC == kPrecBF16x3 ? 2 : 3;
             if constexpr (KQ > 0) {
-                if (quad) {
-                    syrk_tile_split_lds<KQ, PARTS>(A + r + r * lda, lda, Wsplit, slabq / kNB, slot_elems,
-                                                   split_part_elems(n), (int)r, (int)m, ncol, i - c, 0, (char *)smem);
+                if (quad || oct) {
+                    syrk_tile_split_lds<PARTS>(A + r + r * lda, lda, Wsplit, (oct ? slabo : slabq) / kNB, slot_elems,
+                                               split_part_elems(n), oct ? nko : KQ / 32, (int)r, (int)m, ncol, i - c,
+                                               0, (char *)smem);
                     return;
                 }
             }
-            syrk_tile_split_lds<K, PARTS>(A + r + r * lda, lda, Wsplit, slab / kNB, slot_elems, split_part_elems(n),
-                                          (int)r, (int)m, ncol, i - c, 0, (char *)smem);
+            syrk_tile_split_lds<PARTS>(A + r + r * lda, lda, Wsplit, slab / kNB, slot_elems, split_part_elems(n), K / 32,
+                                       (int)r, (int)m, ncol, i - c, 0, (char *)smem);
             return;
         }
     }
@@ -180,41 +192,53 @@ int pair_tiles(int64_t n, int64_t k, int *c0_out)
     return tiles;
 }
 
-QuadPlan quad_plan(int64_t n, int64_t k)
+QuadPlan quad_plan(int64_t n, int64_t k, bool oct)
 {
     QuadPlan q;
     const int nblk = (int)((n + kNB - 1) / kNB);
+    q.oct = oct;
     q.c0 = (int)k + 2;
     q.cq0 = (int)k + 4;
+    q.co0 = (int)k + 8;
     if (k < 0) return q;
     q.npair = q.c0 < nblk ? nblk - q.c0 : 0;
-    for (int c = q.cq0; c < nblk; c += 4) q.nquad += nblk - c;
+    if (oct) {
+        q.nquad = q.cq0 < nblk ? nblk - q.cq0 : 0;
+        for (int c = q.co0; c < nblk; c += 8) q.nocts += nblk - c;
+    } else {
+        for (int c = q.cq0; c < nblk; c += 4) q.nquad += nblk - c;
+    }
     q.K = k >= 1 ? 256 : 128;
     q.slab = (k >= 1 ? k - 1 : 0) * kNB;
     const int64_t first = k >= 3 ? k - 3 : 0;
     q.KQ = (int)(k - first + 1) * kNB;
     q.slabq = first * kNB;
+    const int64_t firsto = k >= 7 ? k - 7 : 0;
+    q.Ko = (int)(k - firsto + 1) * kNB;
+    q.slabo = firsto * kNB;
     return q;
 }
 
 template <typename S, int PREC, int K, int KQ>
 static void launch_one(dim3 grid, hipStream_t s, S *A, int64_t lda, int64_t n, int64_t kp, int Gp, int c0, int nsyrk,
-                       int64_t slab, int cq0, int64_t slabq, double *Wleaf, int *info, uint32_t *tickets,
+                       int64_t slab, int cq0, int64_t slabq, int co0, int64_t slabo, int nko, double *Wleaf, int *info, uint32_t *tickets,
                        uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status, __bf16 *Wsplit,
                        int64_t slot_elems, const Batch &bt)
 {
     hipLaunchKernelGGL((factor_step_kernel<S, PREC, K, KQ>), grid, dim3(256), 0, s, A, lda, n, kp, Gp, c0, nsyrk, slab,
-                       cq0, slabq, Wleaf, info, tickets, ticket_base, flag, epoch, status, Wsplit, slot_elems, bt.a,
+                       cq0, slabq, co0, slabo, nko, Wleaf, info, tickets, ticket_base, flag, epoch, status, Wsplit, slot_elems, bt.a,
                        bt.leaf, bt.sync);
 }
 
 template <typename S, int PREC>
 static void launch_prec(dim3 grid, hipStream_t s, int K, int KQ, S *A, int64_t lda, int64_t n, int64_t kp, int Gp,
-                        int c0, int nsyrk, int64_t slab, int cq0, int64_t slabq, double *Wleaf, int *info,
+                        int c0, int nsyrk, int64_t slab, int cq0, int64_t slabq, int co0, int64_t slabo, int nko,
+                        double *Wleaf, int *info,
                         uint32_t *tickets, uint32_t ticket_base, uint32_t *flag, uint32_t epoch, int *status,
                         __bf16 *Wsplit, int64_t slot_elems, const Batch &bt)
 {
-#define MI_FS_ARGS grid, s, A, lda, n, kp, Gp, c0, nsyrk, slab, cq0, slabq, Wleaf, info, tickets, ticket_base, flag, \
+#define MI_FS_ARGS grid, s, A, lda, n, kp, Gp, c0, nsyrk, slab, cq0, slabq, co0, slabo, nko, Wleaf, info, tickets, \
+                   ticket_base, flag, \
                    epoch, status, Wsplit, slot_elems, bt
     if constexpr (sizeof(S) == 8) {
         if (K == 256)
@@ -240,27 +264,30 @@ void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, dou
                         hipStream_t s, uint32_t *grid_out, int prec, __bf16 *Wsplit, const Batch &bt)
 {
     const int Gp = kp >= 0 ? (int)((n - kp + 127) / 128) : 0;
-    int c0 = 0, nsyrk = 0, K = k >= 1 ? 256 : 128, KQ = 0, cq0 = 0;
-    int64_t slab = k >= 1 ? (k - 1) * kNB : 0, slabq = 0;
+    int c0 = 0, nsyrk = 0, K = k >= 1 ? 256 : 128, KQ = 0, cq0 = 0, co0 = 0, nko = 0;
+    int64_t slab = k >= 1 ? (k - 1) * kNB : 0, slabq = 0, slabo = 0;
     if constexpr (sizeof(S) == 8) {
         nsyrk = pair_tiles(n, k, &c0);
     } else {
-        const QuadPlan q = quad_plan(n, k);
+        const QuadPlan q = quad_plan(n, k, oct_schedule(prec));
         c0 = q.c0;
         cq0 = q.cq0;
-        nsyrk = q.npair + q.nquad;
+        co0 = q.co0;
+        nsyrk = q.npair + q.nquad + q.nocts;
         K = q.K;
         KQ = q.KQ;
+        nko = q.Ko / 32;
         slab = q.slab;
         slabq = q.slabq;
+        slabo = q.slabo;
     }
     const unsigned grid = (unsigned)(Gp + nsyrk);
     *grid_out = grid;
     if (grid == 0) return;
     const dim3 g(grid, 1, bt.count);
     const int64_t slot = split_slot_elems(n, prec == kPrecBF16x6 ? 3 : 2);
-#define MI_FS_ARGS g, s, K, KQ, A, lda, n, kp, Gp, c0, nsyrk, slab, cq0, slabq, Wleaf, info, tickets, ticket_base, \
-                   flag, epoch, status
+#define MI_FS_ARGS g, s, K, KQ, A, lda, n, kp, Gp, c0, nsyrk, slab, cq0, slabq, co0, slabo, nko, Wleaf, info, \
+                   tickets, ticket_base, flag, epoch, status
     if constexpr (sizeof(S) == 8) {
         launch_prec<S, kPrecF64>(MI_FS_ARGS, nullptr, slot, bt);
     } else if (prec == kPrecBF16) {

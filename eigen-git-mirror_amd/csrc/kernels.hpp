@@ -69,8 +69,8 @@ void launch_factor_step(S *A, int64_t lda, int64_t n, int64_t kp, int64_t k, dou
 // for kPrecBF16x3, 3 for kPrecBF16x6) of the split-panel ring (Wsplit, 4 slots).
 __host__ __device__ inline int64_t split_part_elems(int64_t n) { return ((n + 127) / 128 * 128) * 128; }
 // Slots of the ring of split panels: launch k writes panel k+1's slot while its trailing update
-// reads panels k-3..k (mixed-precision schedule), so 5 are in use at once.
-constexpr int kSplitRing = 8;
+// reads panels k-7..k (the 2-part split schedule), so 9 are in use at once.
+constexpr int kSplitRing = 16;
 __host__ __device__ inline int64_t split_slot_elems(int64_t n, int parts) { return parts * split_part_elems(n); }
 // Winv[b] = {X_b^T, X_b, M_b, N_b, M2_b, N2_b}, X_b = L_bb^{-1}, for every diagonal block (needs Wleaf).
 // couplings = false: X_b^T and X_b only (for users of Winv other than the solve).

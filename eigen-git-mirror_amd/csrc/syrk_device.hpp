@@ -44,21 +44,23 @@ __device__ __forceinline__ void syrk_tile(T *__restrict__ C, int64_t ldc, const 
 
 // The split forms (FP32 storage) read from LDS: PARTS = 2 (hi, lo: ~16 significant bits; three
 // products hi.hi, hi.lo, lo.hi) or 3 (hi, mid, lo: all 24 bits; six products hi.hi, hi.mid,
-// mid.hi, hi.lo, lo.hi, mid.mid -- the dropped terms are below 2^-24 of the product).  Depth
-// columns 128 q .. 128 q + 127 are those of the panel in ring slot (sp + q) % kSplitRing (at
-// ring + slot * slot_elems), part t at + t * part_off; C's row 0 is row r0 of the slots.  Each k-step's operands (128 P rows and 128 Q rows x PARTS, 16 KiB
-// per part) are staged once per workgroup in LDS (`lds` >= 32 KiB * PARTS; two waves read
-// every operand row), the next k-step's loads held in registers meanwhile (a raw 16-byte
+// mid.hi, hi.lo, lo.hi, mid.mid -- the dropped terms are below 2^-24 of the product).  The
+// depth is nks k-steps of 32 columns; columns 128 q .. 128 q + 127 are those of the panel in
+// ring slot (sp + q) % kSplitRing (at ring + slot * slot_elems), part t at + t * part_off; C's
+// row 0 is row r0 of the slots.  Each k-step's operands (128 P rows and 128 Q rows x PARTS,
+// 16 KiB per part) are staged once per workgroup in LDS (`lds` >= 32 KiB * PARTS; two waves
+// read every operand row), the next k-step's loads held in registers meanwhile (a raw 16-byte
 // vector type: HIP's uint4 struct would be kept in scratch memory), and the waves read their
 // parts from LDS per half wave tile and a Q part at a time (fewer operand registers).  All
 // four waves take part in the staging, so the strictly upper wave tile of a diagonal tile is
 // computed too (and not stored).
-template <int K, int PARTS>
+template <int PARTS>
 __device__ __forceinline__ void syrk_tile_split_lds(float *__restrict__ C, int64_t ldc, const __bf16 *ring, int64_t sp,
-                                                    int64_t slot_elems, int64_t part_off, int r0, int m, int ncol,
-                                                    int ti, int tj, char *lds)
+                                                    int64_t slot_elems, int64_t part_off, int nks, int r0, int m,
+                                                    int ncol, int ti, int tj, char *lds)
 {
-    constexpr int S = K / 32, NC = 4 * PARTS;  // chunks per thread per k-step
+    constexpr int NC = 4 * PARTS;  // chunks per thread per k-step
+    const int S = nks;
     const int tid = threadIdx.x, l = lane_id();
     const int w = tid >> 6, wr = w >> 1, wc = w & 1;
     // 16-byte chunks (operand P / Q, part, group of 16 rows, lane chunk); thread t moves, for
@@ -90,7 +92,6 @@ __device__ __forceinline__ void syrk_tile_split_lds(float *__restrict__ C, int64
     MI_SL_GLOAD(v, 0);
     MI_SL_LSTORE(v);
     lds_barrier();
-#pragma unroll
     for (int s = 0; s < S; ++s) {
         if (s + 1 < S) MI_SL_GLOAD(v, s + 1);  // in flight during this k-step
 #pragma unroll
