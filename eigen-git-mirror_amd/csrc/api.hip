@@ -209,7 +209,7 @@ int solve_only(mi_llt_handle h, const T *L, int64_t ldl, int64_t n, double *B, i
 {
     int rc = reserve_sync(h, 0);
     if (rc) return rc;
-    const size_t pk = 2 * (size_t)mi::solve_pack_doubles(n, nrhs) * sizeof(double);  // forward, backward
+    const size_t pk = (size_t)mi::solve_pack_doubles(n, nrhs) * sizeof(double);  // both sweeps, in turn
     if (pk > h->ypk_bytes) {
         hipError_t e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return fail_hip(h, e, "hipStreamSynchronize");
@@ -1174,7 +1174,7 @@ int prepare_batch(mi_llt_handle h, int64_t n, int64_t nrhs, int64_t batch, bool 
     const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
     const size_t leaf_d = round_up((size_t)nblk * mi::kPanelDoubles * sizeof(double)) / sizeof(double);
     const size_t inv_d = inv ? round_up((size_t)nblk * mi::kInvDoubles * sizeof(double)) / sizeof(double) : 0;
-    const size_t pack_d = nrhs > 0 ? round_up(2 * (size_t)mi::solve_pack_doubles(n, nrhs) * sizeof(double)) / sizeof(double) : 0;
+    const size_t pack_d = nrhs > 0 ? round_up((size_t)mi::solve_pack_doubles(n, nrhs) * sizeof(double)) / sizeof(double) : 0;
     const size_t aux_d = round_up(aux_doubles * sizeof(double)) / sizeof(double);
     const size_t per = leaf_d + inv_d + pack_d + aux_d;
     int rc = reserve(h, per * (size_t)batch * sizeof(double));

@@ -82,8 +82,9 @@ template <typename T>
 void launch_leaf_inverses(const T *L, int64_t lda, int64_t n, double *Wleaf, hipStream_t s,
                           const Batch &bt = Batch());
 // Solve: forward then backward sweep in place on B (one persistent launch each, each after a
-// small reset launch).  tickets: 2 ints; Ypk: 2 * solve_pack_doubles(n, nrhs) doubles of
-// scratch (each finished block's packed copy for its consumers, per sweep).
+// small reset launch).  tickets: 2 ints; Ypk: solve_pack_doubles(n, nrhs) doubles of scratch
+// (each finished block's packed copy for its consumers), used by the two sweeps in turn: each
+// sweep's reset runs after everything queued before it on the stream.
 // sweeps: 1 forward only (L Y = B), 2 backward only (L^T X = B), 3 both.
 template <typename T>
 // upper_copy: L's strictly upper triangle holds L^T (launch_lower_to_upper), which the backward
@@ -91,8 +92,13 @@ template <typename T>
 void launch_solve_persistent(const T *L, int64_t lda, int64_t n, double *B, int64_t ldb, int64_t nrhs,
                              const double *Winv, double *Ypk, int *tickets, const int *info, int *status, hipStream_t s,
                              int sweeps = 3, bool upper_copy = false, const Batch &bt = Batch());
-// Doubles of one sweep's packed block copies for n rows and nrhs right-hand sides.
-inline int64_t solve_pack_doubles(int64_t n, int64_t nrhs) { return ((n + 127) / 128) * ((nrhs + 15) / 16) * 2048; }
+// Doubles of one sweep's packed copies for n rows and nrhs right-hand sides: per 128-row
+// block and group of 16 right-hand sides, the finished block (128 x 16) and the residual rows
+// that one half of the block hands to the other (64 x 16).
+constexpr int64_t solve_pack_doubles(int64_t n, int64_t nrhs)
+{
+    return ((n + 127) / 128) * ((nrhs + 15) / 16) * (2048 + 1024);
+}
 // Symmetric-storage helpers.
 void launch_upper_to_lower(const double *A, int64_t lda, int64_t n, double *W, int64_t ldw, hipStream_t s);
 // A's upper triangle = W's lower triangle transposed (A may be W: the copy into its own upper

@@ -9,7 +9,7 @@
 //   [ sync words (512 B): status, solve tickets, factor tickets, panel flag, each on its own line
 //   | panel images (kPanelDoubles per 128-block)
 //   | block inverses + couplings (kInvDoubles per 128-block; only when nrhs > 0)
-//   | the solve's packed block copies (2 * solve_pack_doubles(n, nrhs); only when nrhs > 0) ]
+//   | the solve's packed block copies (solve_pack_doubles(n, nrhs); only when nrhs > 0) ]
 // Every call clears what must start at zero (status, tickets, flag; the solve resets its own
 // tickets and packed copies) with small kernels on the caller's stream -- no memset, so the
 // call can be captured and replayed.  All calls on one GPU share that GPU's array: calls that
@@ -69,7 +69,7 @@ extern "C" int mi_dposv_noalloc(char uplo, int64_t n, int64_t nrhs, double *dA, 
     const int64_t nblk = (n + mi::kNB - 1) / mi::kNB;
     const size_t leaf = up256((size_t)nblk * mi::kPanelDoubles * sizeof(double));
     const size_t inv = nrhs > 0 ? up256((size_t)nblk * mi::kInvDoubles * sizeof(double)) : 0;
-    const size_t pack = nrhs > 0 ? up256(2 * (size_t)mi::solve_pack_doubles(n, nrhs) * sizeof(double)) : 0;
+    const size_t pack = nrhs > 0 ? up256((size_t)mi::solve_pack_doubles(n, nrhs) * sizeof(double)) : 0;
     if (kSyncBytes + leaf + inv + pack > kScratchBytes) return MI_LLT_E_TOO_LARGE;
 
     int dev = 0;
