@@ -21,9 +21,14 @@ constexpr int64_t kLeafOffset = 28 * 256;
 // Per diagonal block b, six 128 x 128 matrices (ld 128): X_b^T, X_b (X_b = L_bb^{-1}), then the
 // couplings of the solve M_b = X_b L_{b,b-1}, N_b = X_b^T L_{b+1,b}^T, M2_b = X_b L_{b,b-2},
 // N2_b = X_b^T L_{b+2,b}^T (zero where undefined).
-constexpr int64_t kInvDoubles = 6 * 128 * 128;
+constexpr int64_t kInvDoubles = 8 * 128 * 128;
 constexpr int64_t kInvX = 128 * 128, kInvM = 2 * 128 * 128, kInvN = 3 * 128 * 128;  // offsets
 constexpr int64_t kInvM2 = 4 * 128 * 128, kInvN2 = 5 * 128 * 128;
+constexpr int64_t kInvM3 = 6 * 128 * 128, kInvN3 = 7 * 128 * 128;
+// Couplings of the block substitution per direction (M, M2 and, from 16 blocks on, M3): the
+// third takes a block's triangular product off its predecessor three blocks back, which
+// shortens long chains; for a few blocks it would cost more to build than it saves.
+__host__ __device__ constexpr int solve_couplings(int64_t nblk) { return nblk >= 16 ? 3 : 2; }
 
 // A batched launch: `count` systems of one order, system z (blockIdx.z) at every base pointer
 // plus z times that buffer's stride in elements -- a: the matrix or factor, b: right-hand

@@ -42,14 +42,14 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict_
                        Wleaf + (size_t)b * kPanelDoubles + kLeafOffset, Wout + (size_t)b * kInvDoubles);
 }
 
-// Couplings of the block substitution: with them a block of the solution depends on its two
+// Couplings of the block substitution: with them a block of the solution depends on its three
 // predecessors through one product each,
-//   Y_i = X_i (B_i - sum_{j<i-2} L_ij Y_j) - M2_i Y_{i-2} - M_i Y_{i-1},
-// so everything else is done before they are published.  Workgroup q nblk + b, distance
-// d = 1 + q / 2: q even, M_b / M2_b = X_b L_{b,b-d} (b >= d); q odd, N_b / N2_b =
-// X_b^T L_{b+d,b}^T (b + d < nblk).  128 x 128 each at Winv[b] + kInvM / kInvN / kInvM2 /
-// kInvN2 (ld 128); entries of L outside the matrix read as 0.  4 waves of 64 x 64 (4 x 4 FP64
-// MFMA tiles), depth 128.
+//   Y_i = X_i (B_i - sum_{j<i-3} L_ij Y_j) - M3_i Y_{i-3} - M2_i Y_{i-2} - M_i Y_{i-1},
+// so everything else is done before they are published (the third only from 16 blocks on:
+// solve_couplings).  Workgroup q nblk + b, distance d = 1 + q / 2: q even, M_b / M2_b / M3_b = X_b L_{b,b-d} (b >= d); q odd, N_b / N2_b / N3_b
+// = X_b^T L_{b+d,b}^T (b + d < nblk).  128 x 128 each at Winv[b] + kInvM / kInvN / kInvM2 /
+// kInvN2 / kInvM3 / kInvN3 (ld 128); entries of L outside the matrix read as 0.  4 waves of
+// 64 x 64 (4 x 4 FP64 MFMA tiles), depth 128.
 template <typename T>
 __global__ __launch_bounds__(256) void block_coupling_kernel(const T *__restrict__ L, int64_t ldl, int64_t n,
                                                              double *__restrict__ Winv, const int *info, int64_t sL,
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void block_coupling_kernel(const T *__restrict
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc[a][q] = mfma_f64(qv[q], pv[a], acc[a][q]);
     }
-    double *C = Wb + (fwd ? (d == 1 ? kInvM : kInvM2) : (d == 1 ? kInvN : kInvN2));
+    double *C = Wb + (fwd ? (d == 1 ? kInvM : d == 2 ? kInvM2 : kInvM3) : (d == 1 ? kInvN : d == 2 ? kInvN2 : kInvN3));
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -163,8 +163,8 @@ void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wle
     hipLaunchKernelGGL(block_inverse_kernel<T>, dim3(nblk, 1, bt.count), dim3(256), 0, s, L, lda, n, Wleaf, Winv,
                        info, bt.a, bt.leaf, bt.inv);
     if (couplings && nblk > 1)
-        hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(4 * nblk, 1, bt.count), dim3(256), 0, s, L, lda, n, Winv,
-                           info, bt.a, bt.inv);
+        hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(2 * solve_couplings(nblk) * nblk, 1, bt.count), dim3(256), 0,
+                           s, L, lda, n, Winv, info, bt.a, bt.inv);
 }
 
 template <typename T>

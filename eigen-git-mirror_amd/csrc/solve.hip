@@ -110,19 +110,24 @@ __device__ __forceinline__ bool fetch_published(const double *src, int *status, 
 
 // BACKWARD = false: forward sweep with L; true: backward sweep with L^T.
 // Winv holds, per block b (at Winv + b * kInvDoubles, each ld 128): X_b^T, X_b, and the
-// couplings M_b = X_b L_{b,b-1}, N_b = X_b^T L_{b+1,b}^T.  Ypk: per group of right-hand
-// sides, nblk packed blocks, then per group nblk packed half blocks (the residual rows that
-// the other half of the block needs), all filled with kUnwritten before the launch.
+// couplings M_b, M2_b, M3_b = X_b L_{b,b-d} and N_b, N2_b, N3_b = X_b^T L_{b+d,b}^T (d = 1, 2,
+// 3).  Ypk: per group of right-hand sides, nblk packed blocks, then per group nblk packed half
+// blocks (the first half's share Q of the other half's rows, below), all filled with kUnwritten
+// before the launch.
 // T: storage of L (float for a mixed-precision factor, widened on load; FP64 arithmetic).
 //
-// Block i depends on its two predecessors (i-1, i-2 forward; i+1, i+2 backward) through one
-// product each:
-//   forward : Y_i = Z_i - M2_i Y_{i-2} - M_i Y_{i-1},  Z_i = X_i (B_i - sum_{j<i-2} L_ij Y_j)
-//   backward: X_i = Z_i - N2_i X_{i+2} - N_i X_{i+1},  Z_i = X_i^T (Y_i - sum_{j>i+2} L_ji^T X_j)
-// so a workgroup finishes Z_i (its triangular X_i product included) two blocks ahead, the
-// i-2 product one block ahead, and the step from one block to the next is: see i-1, stage
-// it, 32 MFMA steps (on four independent chains), store.  Two workgroups share a block, 64
-// rows each, so that step's MFMAs (one CU's FP64 rate) are half those of a whole block.
+// Block i depends on its three predecessors (i-1, i-2, i-3 forward; i+1, i+2, i+3 backward)
+// through one product each:
+//   forward : Y_i = Z_i - M3_i Y_{i-3} - M2_i Y_{i-2} - M_i Y_{i-1},
+//             Z_i = X_i (B_i - sum_{j<i-3} L_ij Y_j)
+//   backward: X_i = Z_i - N3_i X_{i+3} - N2_i X_{i+2} - N_i X_{i+1},
+//             Z_i = X_i^T (Y_i - sum_{j>i+3} L_ji^T X_j)
+// (with fewer than 16 blocks, two: no M3_i, and Z_i sums to j < i-2), so a workgroup finishes
+// Z_i (its triangular X_i product included) three blocks ahead, the i-3 and i-2 products two
+// and one block ahead, and the step from one block to the next is:
+// see i-1, stage it, 32 MFMA steps (on four independent chains), store.  Two workgroups share
+// a block, 64 rows each, so that step's MFMAs (one CU's FP64 rate) are half those of a whole
+// block.
 //
 // X_i is lower triangular (X_i^T upper), so the rows of one half of Z_i need only that half's
 // residual rows T: the first half forward, the second backward.  That half (taken first from
@@ -198,10 +203,12 @@ __global__ __launch_bounds__(ST, 2) void solve_sweep_kernel(const T *__restrict_
     // the predecessors and the couplings to them (P[row][k] = M_i[row][k], M2_i[row][k], ...)
     const int jstep = BACKWARD ? -1 : 1;
     const int npred = BACKWARD ? nblk - 1 - ib : ib;  // blocks before this one in the sweep
-    const bool crit1 = npred >= 1, crit2 = npred >= 2;
-    const int jend = ib - jstep * (npred < 2 ? npred : 2);  // plain contributions: [start, jend)
+    const int ncp = solve_couplings(nblk);  // coupled predecessors: 3, or 2 for few blocks
+    const bool crit1 = npred >= 1, crit2 = npred >= 2, crit3 = npred >= 3 && ncp == 3;
+    const int jend = ib - jstep * (npred < ncp ? npred : ncp);  // plain contributions: [start, jend)
     const double *M1p = Wi + (BACKWARD ? kInvN : kInvM) + r + SB * kl;
     const double *M2p = Wi + (BACKWARD ? kInvN2 : kInvM2) + r + SB * kl;
+    const double *M3p = Wi + (BACKWARD ? kInvN3 : kInvM3) + r + SB * kl;
 
     // Stages packed block jb into Ys once it is written (false: given up).
     auto stage = [&](int jb) -> bool {
@@ -264,7 +271,7 @@ __global__ __launch_bounds__(ST, 2) void solve_sweep_kernel(const T *__restrict_
     auto x_slot = [&](int s) { pv[s] = (s < 16 ? Xlo : Xhi)[(size_t)4 * SB * s]; };
     dvec4 acc = {0.0, 0.0, 0.0, 0.0};
     int jb = BACKWARD ? nblk - 1 : 0;
-    const bool contrib = jb != jend;  // any plain contributions (before the two coupled blocks)
+    const bool contrib = jb != jend;  // any plain contributions (before the three coupled blocks)
     if (contrib) {
         aim(jb);
         load_operand(pv);
@@ -311,7 +318,7 @@ __global__ __launch_bounds__(ST, 2) void solve_sweep_kernel(const T *__restrict_
     // only with crit1): a load in a branch would make the compiler's count of loads in flight
     // inexact, and every later MFMA would wait for all of them.
     constexpr int HQ = BACKWARD ? 1 : 0;  // the first half (h of an item with indep)
-    const double *Cp = crit2 ? M2p : M1p;
+    const double *Cp = crit3 ? M3p : crit2 ? M2p : M1p;
     dvec4 y = {0.0, 0.0, 0.0, 0.0}, yq = {0.0, 0.0, 0.0, 0.0};
     __builtin_amdgcn_s_setprio(1);  // the rest of this item leads to the chain
 #pragma unroll
@@ -354,11 +361,24 @@ __global__ __launch_bounds__(ST, 2) void solve_sweep_kernel(const T *__restrict_
 #pragma unroll
     for (int e = 0; e < 4; ++e) qv[e] = coherent_load(qsrc + 4 * e);
 
-    // the coupling products, y -= (M2_i or N2_i) Y_{i-2}, then y -= (M_i or N_i) Y_{i-1}, each on
-    // four independent MFMA chains; the second's operand loads follow the first's MFMAs.  The
-    // i-2 product comes one block before the chain needs it (its waves keep the raised issue
-    // priority over the other item resident on this CU, whose contributions are further ahead).
+    // the coupling products, y -= (M3_i or N3_i) Y_{i-3}, y -= (M2_i or N2_i) Y_{i-2}, then
+    // y -= (M_i or N_i) Y_{i-1}, each on four independent MFMA chains; each one's operand loads
+    // follow the previous one's MFMAs.  The i-3 and i-2 products come before the chain needs
+    // them (their waves keep the raised issue priority over the other item resident on this CU,
+    // whose contributions are further ahead).
     dvec4 cq[4];
+    if (crit3) {
+        if (!stage(ib - 3 * jstep)) return;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cq[q] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < SB / 4; ++s) {
+            cq[s & 3] = mfma_f64(ys(4 * s + kl), pv[s], cq[s & 3]);
+            pv[s] = M2p[(size_t)4 * SB * s];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) y[e] -= (cq[0][e] + cq[1][e]) + (cq[2][e] + cq[3][e]);
+    }
     if (crit2) {
         if (!stage(ib - 2 * jstep)) return;
 #pragma unroll

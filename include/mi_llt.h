@@ -130,14 +130,14 @@ int mi_dposv_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, double *
  * hipStream_t, NULL: the null stream), without a handle: every scratch buffer lives in one
  * fixed device array of MI_NOALLOC_SCRATCH_BYTES per GPU (held from the library's first use on
  * that GPU until the process ends), and the call allocates nothing.  The problem must fit it:
- *   256 + 73728 nb + (nrhs > 0 ? 786432 nb + 24576 nb ceil(nrhs/16) : 0) bytes, nb = ceil(n/128)
+ *   256 + 73728 nb + (nrhs > 0 ? 1048576 nb + 24576 nb ceil(nrhs/16) : 0) bytes, nb = ceil(n/128)
  * (e.g. n = 8192 with up to 112 right-hand sides, n = 8064 with 128; a factor alone, nrhs = 0,
- * up to n = 116480); larger problems return MI_LLT_E_TOO_LARGE and queue nothing.  uplo 'U'
+ * up to n = 145536); larger problems return MI_LLT_E_TOO_LARGE and queue nothing.  uplo 'U'
  * returns MI_LLT_E_UNSUPPORTED.  d_info (device int): 0, k+1 for a failing pivot at column k
  * (B then unchanged), or MI_LLT_E_DEVICE when the device recorded a failure (a timeout).  All
  * calls on one GPU share its array: calls that overlap on one GPU (from two streams or two
  * threads) race and give wrong results.  Asynchronous. */
-#define MI_NOALLOC_SCRATCH_BYTES (64ll << 20)
+#define MI_NOALLOC_SCRATCH_BYTES (80ll << 20)
 #define MI_LLT_E_TOO_LARGE (-1006) /* the problem needs more than the fixed scratch array holds */
 int mi_dposv_noalloc(char uplo, int64_t n, int64_t nrhs, double *dA, int64_t lda, double *dB, int64_t ldb,
                      int *d_info, void *stream);

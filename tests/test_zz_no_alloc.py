@@ -83,5 +83,5 @@ def test_noalloc_failing_pivot_and_errors(dev):
     assert f(b"L", n, r, p, n, vp(b.data_ptr()), n, vp(None), s) == -8
     # beyond the fixed scratch: rejected before anything is queued (the pointers are never used)
     assert f(b"L", 8192, 113, p, 8192, vp(b.data_ptr()), 8192, vp(d.data_ptr()), s) == E_TOO_LARGE
-    assert f(b"L", 116608, 0, p, 116608, vp(b.data_ptr()), 116608, vp(d.data_ptr()), s) == E_TOO_LARGE
+    assert f(b"L", 145664, 0, p, 145664, vp(b.data_ptr()), 145664, vp(d.data_ptr()), s) == E_TOO_LARGE
     torch.cuda.synchronize()
