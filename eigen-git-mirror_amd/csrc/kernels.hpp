@@ -25,10 +25,12 @@ constexpr int64_t kInvDoubles = 8 * 128 * 128;
 constexpr int64_t kInvX = 128 * 128, kInvM = 2 * 128 * 128, kInvN = 3 * 128 * 128;  // offsets
 constexpr int64_t kInvM2 = 4 * 128 * 128, kInvN2 = 5 * 128 * 128;
 constexpr int64_t kInvM3 = 6 * 128 * 128, kInvN3 = 7 * 128 * 128;
-// Couplings of the block substitution per direction (M, M2 and, from 16 blocks on, M3): the
-// third takes a block's triangular product off its predecessor three blocks back, which
-// shortens long chains; for a few blocks it would cost more to build than it saves.
-__host__ __device__ constexpr int solve_couplings(int64_t nblk) { return nblk >= 16 ? 3 : 2; }
+// Couplings of the block substitution per direction: M, M2 from 8 blocks on, and M3 from 16.
+// Each takes a block's triangular product one predecessor further off the chain from block
+// to block, which shortens long chains; for a few blocks (a batch of small systems, whose
+// chains are short and whose launch is bound by throughput) they cost more to build than they
+// save, and the substitution is plain.
+__host__ __device__ constexpr int solve_couplings(int64_t nblk) { return nblk >= 16 ? 3 : nblk >= 8 ? 2 : 0; }
 
 // A batched launch: `count` systems of one order, system z (blockIdx.z) at every base pointer
 // plus z times that buffer's stride in elements -- a: the matrix or factor, b: right-hand

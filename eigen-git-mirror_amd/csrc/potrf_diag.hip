@@ -45,8 +45,8 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict_
 // Couplings of the block substitution: with them a block of the solution depends on its three
 // predecessors through one product each,
 //   Y_i = X_i (B_i - sum_{j<i-3} L_ij Y_j) - M3_i Y_{i-3} - M2_i Y_{i-2} - M_i Y_{i-1},
-// so everything else is done before they are published (the third only from 16 blocks on:
-// solve_couplings).  Workgroup q nblk + b, distance d = 1 + q / 2: q even, M_b / M2_b / M3_b = X_b L_{b,b-d} (b >= d); q odd, N_b / N2_b / N3_b
+// so everything else is done before they are published (solve_couplings: none below 8 blocks,
+// the third only from 16 blocks on).  Workgroup q nblk + b, distance d = 1 + q / 2: q even, M_b / M2_b / M3_b = X_b L_{b,b-d} (b >= d); q odd, N_b / N2_b / N3_b
 // = X_b^T L_{b+d,b}^T (b + d < nblk).  128 x 128 each at Winv[b] + kInvM / kInvN / kInvM2 /
 // kInvN2 / kInvM3 / kInvN3 (ld 128); entries of L outside the matrix read as 0.  4 waves of
 // 64 x 64 (4 x 4 FP64 MFMA tiles), depth 128.
@@ -162,7 +162,7 @@ void launch_block_inverses(const T *L, int64_t lda, int64_t n, const double *Wle
     const unsigned nblk = (unsigned)((n + 127) / 128);
     hipLaunchKernelGGL(block_inverse_kernel<T>, dim3(nblk, 1, bt.count), dim3(256), 0, s, L, lda, n, Wleaf, Winv,
                        info, bt.a, bt.leaf, bt.inv);
-    if (couplings && nblk > 1)
+    if (couplings && solve_couplings(nblk) > 0)
         hipLaunchKernelGGL(block_coupling_kernel<T>, dim3(2 * solve_couplings(nblk) * nblk, 1, bt.count), dim3(256), 0,
                            s, L, lda, n, Winv, info, bt.a, bt.inv);
 }

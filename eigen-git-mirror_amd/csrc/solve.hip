@@ -122,7 +122,8 @@ __device__ __forceinline__ bool fetch_published(const double *src, int *status, 
 //             Z_i = X_i (B_i - sum_{j<i-3} L_ij Y_j)
 //   backward: X_i = Z_i - N3_i X_{i+3} - N2_i X_{i+2} - N_i X_{i+1},
 //             Z_i = X_i^T (Y_i - sum_{j>i+3} L_ji^T X_j)
-// (with fewer than 16 blocks, two: no M3_i, and Z_i sums to j < i-2), so a workgroup finishes
+// (with fewer than 16 blocks two, no M3_i, and Z_i sums to j < i-2; with fewer than 8 none, and
+// Y_i = Z_i sums over all j < i), so a workgroup finishes
 // Z_i (its triangular X_i product included) three blocks ahead, the i-3 and i-2 products two
 // and one block ahead, and the step from one block to the next is:
 // see i-1, stage it, 32 MFMA steps (on four independent chains), store.  Two workgroups share
@@ -203,8 +204,8 @@ __global__ __launch_bounds__(ST, 2) void solve_sweep_kernel(const T *__restrict_
     // the predecessors and the couplings to them (P[row][k] = M_i[row][k], M2_i[row][k], ...)
     const int jstep = BACKWARD ? -1 : 1;
     const int npred = BACKWARD ? nblk - 1 - ib : ib;  // blocks before this one in the sweep
-    const int ncp = solve_couplings(nblk);  // coupled predecessors: 3, or 2 for few blocks
-    const bool crit1 = npred >= 1, crit2 = npred >= 2, crit3 = npred >= 3 && ncp == 3;
+    const int ncp = solve_couplings(nblk);  // coupled predecessors: 3, 2, or 0 for few blocks
+    const bool crit1 = npred >= 1 && ncp >= 1, crit2 = npred >= 2 && ncp >= 2, crit3 = npred >= 3 && ncp >= 3;
     const int jend = ib - jstep * (npred < ncp ? npred : ncp);  // plain contributions: [start, jend)
     const double *M1p = Wi + (BACKWARD ? kInvN : kInvM) + r + SB * kl;
     const double *M2p = Wi + (BACKWARD ? kInvN2 : kInvM2) + r + SB * kl;
