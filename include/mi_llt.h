@@ -260,7 +260,7 @@ int mi_dpotrs_adjoint(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, const
  * mi_dpotrf_batched_async: d_info is a device array of `batch` ints (LAPACK info per system).
  * mi_dpotrs_batched: solve with each system's factor as left by mi_dpotrf_batched_async.
  * mi_dpotrf_adjoint_batched / mi_dpotrs_adjoint_batched: mi_dpotrf_adjoint / mi_dpotrs_adjoint
- *   per system.  Device workspace per system: 0.82 MiB per 128 rows (panel images and block
+ *   per system.  Device workspace per system: 1.07 MiB per 128 rows (panel images and block
  *   inverses) plus 3 n^2 doubles (factor adjoint) or 2 n nrhs doubles and the solve's packed
  *   copies (solve adjoint). */
 int mi_dpotrf_batched_async(mi_llt_handle h, char uplo, int64_t n, int64_t batch, double *dA, int64_t lda,
