@@ -33,7 +33,7 @@ def stack(n, batch, seed, ld=None, pad=0):
 
 
 @pytest.mark.parametrize("n,batch,nrhs", [(1, 3, 1), (100, 5, 2), (129, 4, 3), (300, 7, 16), (512, 6, 1),
-                                          (640, 3, 17)])
+                                          (640, 3, 17), (1000, 2, 5), (2049, 2, 3)])
 def test_batched_factor_and_solve_match_single(dev, n, batch, nrhs):
     say(f"[gpu] batched potrf/potrs n={n} batch={batch} nrhs={nrhs}")
     a, mats = stack(n, batch, seed=10 * n)

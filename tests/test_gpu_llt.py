@@ -114,7 +114,11 @@ def test_upper_matches_golden_and_oracle(dev, n):
         np.testing.assert_allclose(np.triu(m), np.triu(g[f"upper{n}/LLT"]), rtol=0, atol=TOL_L)
 
 
-@pytest.mark.parametrize("n,r", [(1, 1), (100, 1), (129, 3), (300, 16), (513, 64), (640, 65), (257, 130)])
+# The solve's chain takes 0, 2 or 3 coupled predecessors by its block count (below 8 blocks,
+# 8-15, 16 and more): 1000 (8 blocks, last one 104 rows), 2049 (17 blocks, last one a single
+# row, so one 64-row half of it is empty) and 2176 (17 whole blocks) reach the coupled paths.
+@pytest.mark.parametrize("n,r", [(1, 1), (100, 1), (129, 3), (300, 16), (513, 64), (640, 65), (257, 130),
+                                 (1000, 17), (2049, 40), (2176, 1)])
 def test_solve_matches_oracle(dev, n, r):
     say(f"[gpu] solve n={n} r={r}")
     a = workloads.gram_spd(n, seed=n + 1)
