@@ -130,7 +130,7 @@ int mi_dposv_async(mi_llt_handle h, char uplo, int64_t n, int64_t nrhs, double *
  * hipStream_t, NULL: the null stream), without a handle: every scratch buffer lives in one
  * fixed device array of MI_NOALLOC_SCRATCH_BYTES per GPU (held from the library's first use on
  * that GPU until the process ends), and the call allocates nothing.  The problem must fit it:
- *   256 + 73728 nb + (nrhs > 0 ? 1048576 nb + 24576 nb ceil(nrhs/16) : 0) bytes, nb = ceil(n/128)
+ *   512 + 73728 nb + (nrhs > 0 ? 1048576 nb + 24576 nb ceil(nrhs/16) : 0) bytes, nb = ceil(n/128)
  * (e.g. n = 8192 with up to 112 right-hand sides, n = 8064 with 128; a factor alone, nrhs = 0,
  * up to n = 145536); larger problems return MI_LLT_E_TOO_LARGE and queue nothing.  uplo 'U'
  * returns MI_LLT_E_UNSUPPORTED.  d_info (device int): 0, k+1 for a failing pivot at column k
