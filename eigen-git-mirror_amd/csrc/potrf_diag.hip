@@ -19,7 +19,28 @@
 
 namespace mi {
 
+// Packed lower 16 x 16 tiles of a 128 x 128 block in LDS, the diagonal included: tile (ti, tj),
+// ti >= tj, at 256 (ti (ti + 1) / 2 + tj), column-major.
+__device__ __forceinline__ int tile_at(int ti, int tj) { return 256 * ((ti * (ti + 1)) / 2 + tj); }
+
+// acc += A B for 16 x 16 tiles at LDS offsets a and bb (column-major): the MFMA operands are
+// P[i][k] = A[i][k] and Q[j][k] = B[k][j], k = 4s + lane/16; acc[r] is (lane%16, 4r + lane/16).
+__device__ __forceinline__ void tile_mma(const double *S, int a, int bb, dvec4 &acc)
+{
+    const int l = lane_id(), i = l & 15, kl = l >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = mfma_f64(S[bb + 4 * s + kl + 16 * i], S[a + i + 16 * (4 * s + kl)], acc);
+}
+
 // Workgroup b: X_b^T = L_bb^{-T} to Wout + b*kInvDoubles (ld 128) and X_b at + kInvX.
+// A triangular inverse by a tree over the diagonal 16 x 16 leaves (the recursive form of
+// LAPACK's dtrtri): from the 8 leaf inverses X_pp of the factorization, three levels double
+// the inverted diagonal blocks, X_BA = -X_BB (L_BA X_AA) for [A; B] of 16 + 16, 32 + 32 and
+// 64 + 64 rows, each level's products over 16 x 16 tiles on FP64 MFMA, in place in LDS (L_BA
+// is replaced by the product T = L_BA X_AA, then by X_BA).  Three levels of two products each
+// instead of eight dependent 16-column solve steps: the kernel is bound by one workgroup's
+// latency, not by its flops.  Beyond bs (a last block narrower than 128) L reads as 0 and the
+// leaf inverses are the identity's, so X is padded with the identity.
 // Batched launches (every kernel of this file): system z = blockIdx.z, its pointers advanced
 // by z times their strides (s*), its info at info + z.
 template <typename T>
@@ -28,7 +49,7 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict_
                                                             double *__restrict__ Wout, const int *info, int64_t sL,
                                                             int64_t sWl, int64_t sWo)
 {
-    __shared__ double smem[kSmem];
+    __shared__ double S[kSmem];  // the 36 packed lower tiles
     const int64_t z = blockIdx.z;
     L += z * sL;
     Wleaf += z * sWl;
@@ -38,8 +59,101 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const T *__restrict_
     const int b = (int)blockIdx.x;
     const int64_t kb = (int64_t)b * 128;
     const int bs = (int)((n - kb) < 128 ? (n - kb) : 128);
-    trsm_rows<true, T>(smem, (T *)nullptr, 0, 128, L + kb + kb * ldl, ldl, bs,
-                       Wleaf + (size_t)b * kPanelDoubles + kLeafOffset, Wout + (size_t)b * kInvDoubles);
+    const T *Lb = L + kb + kb * ldl;
+    const double *Xl = Wleaf + (size_t)b * kPanelDoubles + kLeafOffset;
+    const int tid = threadIdx.x, l = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    // the strictly lower tiles of L_bb (0 beyond bs) and the leaf inverses on the diagonal: a
+    // thread takes row r, columns of parity tid / 128, its loads issued 16 at a time (from
+    // clamped, in-bounds addresses) before their LDS stores
+    {
+        const int r = tid & 127, ch = tid >> 7, rt = r >> 4;
+        double x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = Xl[tid + 256 * q];
+#pragma unroll
+        for (int q0 = 0; q0 < 64; q0 += 16) {
+            double v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int c = 2 * (q0 + q) + ch;
+                const bool in = r < bs && c < bs;
+                v[q] = (double)Lb[(in ? r : 0) + (int64_t)(in ? c : 0) * ldl];
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int c = 2 * (q0 + q) + ch;
+                const bool in = r < bs && c < bs;
+                if ((c >> 4) < rt) S[tile_at(rt, c >> 4) + (r & 15) + 16 * (c & 15)] = in ? v[q] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {  // element tid of leaf q: (tid % 16, tid / 16)
+            const int i = tid & 15, j = tid >> 4;
+            S[tile_at(q, q) + tid] = j <= i ? x[q] : 0.0;
+        }
+    }
+    __syncthreads();
+
+    // level v: pairs of sz-tile blocks A (tiles A0 ..) and B (tiles A0 + sz ..); 4 sz output
+    // tiles (B0 + ti, A0 + tj) per level, dealt over the 4 waves
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const int sz = 1 << v, nout = 4 * sz;
+        dvec4 acc[4];
+        // T = L_BA X_AA (X_AA lower: k >= tj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc[q] = dvec4{0.0, 0.0, 0.0, 0.0};
+            const int o = w + 4 * q;
+            if (o < nout) {
+                const int A0 = (o / (sz * sz)) * 2 * sz, B0 = A0 + sz, ti = (o % (sz * sz)) / sz, tj = o % sz;
+                for (int k = tj; k < sz; ++k) tile_mma(S, tile_at(B0 + ti, A0 + k), tile_at(A0 + k, A0 + tj), acc[q]);
+            }
+        }
+        __syncthreads();  // every wave has read L_BA
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int o = w + 4 * q;
+            if (o < nout) {
+                const int A0 = (o / (sz * sz)) * 2 * sz, B0 = A0 + sz, ti = (o % (sz * sz)) / sz, tj = o % sz;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) S[tile_at(B0 + ti, A0 + tj) + (l & 15) + 16 * (4 * r + (l >> 4))] = acc[q][r];
+            }
+        }
+        __syncthreads();
+        // X_BA = -X_BB T (X_BB lower: k <= ti)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc[q] = dvec4{0.0, 0.0, 0.0, 0.0};
+            const int o = w + 4 * q;
+            if (o < nout) {
+                const int A0 = (o / (sz * sz)) * 2 * sz, B0 = A0 + sz, ti = (o % (sz * sz)) / sz, tj = o % sz;
+                for (int k = 0; k <= ti; ++k) tile_mma(S, tile_at(B0 + ti, B0 + k), tile_at(B0 + k, A0 + tj), acc[q]);
+            }
+        }
+        __syncthreads();  // every wave has read T
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int o = w + 4 * q;
+            if (o < nout) {
+                const int A0 = (o / (sz * sz)) * 2 * sz, B0 = A0 + sz, ti = (o % (sz * sz)) / sz, tj = o % sz;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) S[tile_at(B0 + ti, A0 + tj) + (l & 15) + 16 * (4 * r + (l >> 4))] = -acc[q][r];
+            }
+        }
+        __syncthreads();
+    }
+
+    // X to Wout + kInvX and X^T to Wout (ld 128), zeros above the diagonal
+    double *Wt = Wout + (size_t)b * kInvDoubles, *Wx = Wt + kInvX;
+    for (int e = tid; e < 128 * 128; e += 256) {
+        const int r = e & 127, c = e >> 7;
+        const double x = (r >> 4) >= (c >> 4) ? S[tile_at(r >> 4, c >> 4) + (r & 15) + 16 * (c & 15)] : 0.0;
+        Wx[r + 128 * c] = x;
+        Wt[c + 128 * r] = x;
+    }
 }
 
 // Couplings of the block substitution: with them a block of the solution depends on its three
