@@ -293,17 +293,15 @@ __device__ __forceinline__ void stage_panel_image(double *smem, const double *__
 }
 
 // Rows 0..127 of P (valid m) <- P * L^{-T} with L and its 8 leaf inverses Xl (read from
-// global into the LDS panel image).  Each wave owns 32 rows.  IDENTITY: P is the identity (not
-// read); the result L^{-T} goes to O (ld 128) and its transpose L^{-1} to O + 128*128.  T =
-// float: P and L are read as FP32 and widened, the arithmetic is FP64, the result is rounded
-// back to FP32.  If IMAGE is given, the panel image is copied from there instead of built from
+// global into the LDS panel image).  Each wave owns 32 rows.  T = float: P and L are read as
+// FP32 and widened, the arithmetic is FP64, the result is rounded back to FP32.  If IMAGE is given, the panel image is copied from there instead of built from
 // L and Xl.  If SPLIT (a split-panel slot) is given (T = float), the rounded rows are also
 // written split into split_parts BF16 parts (2 or 3), row i as row split_row0 + i of the slot
 // (split_store_row; part t at split + t * split_lo).
-template <bool IDENTITY, typename T>
+template <typename T>
 __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64_t ldp, int m,
                                           const T *__restrict__ L, int64_t ldl, int bs,
-                                          const double *__restrict__ Xl, double *__restrict__ O,
+                                          const double *__restrict__ Xl,
                                           const double *__restrict__ image = nullptr, __bf16 *split = nullptr,
                                           int64_t split_lo = 0, int64_t split_row0 = 0, int split_parts = 2)
 {
@@ -318,12 +316,8 @@ __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int j = 16 * p + 4 * r + (l >> 4);
-                if (IDENTITY) {
-                    X[t][p][r] = (i == j) ? 1.0 : 0.0;
-                } else {
-                    const double pv = (double)P[(i < m ? i : m - 1) + (int64_t)j * ldp];
-                    X[t][p][r] = (i < m) ? pv : 0.0;
-                }
+                const double pv = (double)P[(i < m ? i : m - 1) + (int64_t)j * ldp];
+                X[t][p][r] = (i < m) ? pv : 0.0;
             }
     }
     if (image)
@@ -335,20 +329,12 @@ __device__ __forceinline__ void trsm_rows(double *smem, T *__restrict__ P, int64
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int i = row0 + 16 * t + (l & 15);
-        if (IDENTITY || i < m) {
+        if (i < m) {
 #pragma unroll
             for (int p = 0; p < 8; ++p)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int j = 16 * p + 4 * r + (l >> 4);
-                    if (IDENTITY) {
-                        O[i + (int64_t)j * 128] = X[t][p][r];
-                        O[128 * 128 + j + 128 * i] = X[t][p][r];
-                    } else {
-                        P[i + (int64_t)j * ldp] = (T)X[t][p][r];
-                    }
-                }
-            if constexpr (!IDENTITY && sizeof(T) == 4) {
+                for (int r = 0; r < 4; ++r) P[i + (int64_t)(16 * p + 4 * r + (l >> 4)) * ldp] = (T)X[t][p][r];
+            if constexpr (sizeof(T) == 4) {
                 if (split) {
                     float v[8][4];
 #pragma unroll
@@ -566,7 +552,7 @@ __device__ __forceinline__ void panel_role(int g, double *smem, int *s_ok, T *__
         }
         global_barrier();  // this role's updated rows are in L2 before other waves read them back
         if (!wait_panel_flag(s_ok, flag, epoch, status) || factor_failed(info)) return;
-        trsm_rows<false, T>(smem, Ck, lda, rows, (const T *)nullptr, 0, 128, nullptr, nullptr, Wp, split, split_lo,
+        trsm_rows<T>(smem, Ck, lda, rows, (const T *)nullptr, 0, 128, nullptr, Wp, split, split_lo,
                             R0, PREC == kPrecBF16x6 ? 3 : 2);
     }
 }

@@ -14,7 +14,8 @@
 //   * the panel solve A21 <- A21 L11^{-T} (TriangularSolverMatrix.h:186-329) is blocked by
 //     16 columns with the leaf inverses; the MFMA result layout is the operand layout of the
 //     next k-step, so solved blocks feed the next MFMAs straight from registers.
-// Run on an identity panel, the same solve yields L_bb^{-T}: block_inverse_kernel.
+// The solve's block inverses L_bb^{-1} are built after the factor from its leaf inverses
+// (block_inverse_kernel), and the couplings from them (block_coupling_kernel).
 #include "potrf_device.hpp"
 
 namespace mi {
